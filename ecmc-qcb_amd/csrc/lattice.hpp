@@ -1,0 +1,110 @@
+// lattice.hpp -- periodic 4-D lattice indexing and the structure-of-arrays
+// link layout in HBM.
+//
+// Site index s = ((t*Lz + z)*Ly + y)*Lx + x (x fastest), as GeometryCB::index
+// (src/geometry/GeometryCB.h:25-27) but over the whole periodic lattice, with
+// neighbours computed on the fly instead of the reference's 8-entry table plus
+// width-1 halos (src/geometry/GeometryCB.cpp:10-31,
+// src/mpi/HalosExchange.cpp:4-72).  Extents need not be powers of two.
+//
+// Link field layout (FP64 path): complex entry k = 3i+j of U_mu(s) lives at
+//   U[(mu*9 + k)*V + s]   (one double2 = (re, im) per entry),
+// so a wavefront reading 64 consecutive sites issues 16-byte coalesced loads.
+// su(3) fields (force / RK3 accumulator) hold 8 reals per link as 4 double2:
+//   A[(mu*4 + k)*V + s].
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "su3.hpp"
+
+namespace qcb {
+
+struct lattice {
+    int L[4];           // Lx, Ly, Lz, Lt
+    unsigned V;         // number of sites
+    unsigned stride[4]; // 1, Lx, Lx*Ly, Lx*Ly*Lz
+};
+
+inline lattice make_lattice(const int dims[4]) {
+    lattice g;
+    unsigned st = 1;
+    for (int d = 0; d < 4; d++) {
+        g.L[d] = dims[d];
+        g.stride[d] = st;
+        st *= (unsigned)dims[d];
+    }
+    g.V = st;
+    return g;
+}
+
+// Signed index offsets of the eight nearest neighbours of one site.
+struct nbr {
+    int up[4];
+    int dn[4];
+};
+
+__device__ inline nbr neighbours(const lattice& g, unsigned s) {
+    unsigned c[4];
+    unsigned r = s;
+    c[0] = r % (unsigned)g.L[0];
+    r /= (unsigned)g.L[0];
+    c[1] = r % (unsigned)g.L[1];
+    r /= (unsigned)g.L[1];
+    c[2] = r % (unsigned)g.L[2];
+    c[3] = r / (unsigned)g.L[2];
+    nbr n;
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+        const int st = (int)g.stride[d];
+        const int wrap = (g.L[d] - 1) * st;
+        n.up[d] = (c[d] == (unsigned)(g.L[d] - 1)) ? -wrap : st;
+        n.dn[d] = (c[d] == 0u) ? wrap : -st;
+    }
+    return n;
+}
+
+// ---- FP64 structure-of-arrays link field -----------------------------------
+struct field64 {
+    const double2* __restrict__ p;
+    unsigned V;
+    __device__ inline mat3<double> link(int mu, unsigned s) const {
+        mat3<double> m;
+        const double2* b = p + (size_t)mu * 9 * V + s;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            double2 v = b[(size_t)k * V];
+            m.e[k / 3][k % 3] = {v.x, v.y};
+        }
+        return m;
+    }
+};
+
+__device__ inline void store_link64(double2* __restrict__ p, unsigned V, int mu, unsigned s,
+                                    const mat3<double>& m) {
+    double2* b = p + (size_t)mu * 9 * V + s;
+#pragma unroll
+    for (int k = 0; k < 9; k++) b[(size_t)k * V] = make_double2(m.e[k / 3][k % 3].r, m.e[k / 3][k % 3].i);
+}
+
+__device__ inline alg8<double> load_alg64(const double2* __restrict__ p, unsigned V, int mu,
+                                          unsigned s) {
+    alg8<double> a;
+    const double2* b = p + (size_t)mu * 4 * V + s;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        double2 v = b[(size_t)k * V];
+        a.a[2 * k] = v.x;
+        a.a[2 * k + 1] = v.y;
+    }
+    return a;
+}
+
+__device__ inline void store_alg64(double2* __restrict__ p, unsigned V, int mu, unsigned s,
+                                   const alg8<double>& a) {
+    double2* b = p + (size_t)mu * 4 * V + s;
+#pragma unroll
+    for (int k = 0; k < 4; k++) b[(size_t)k * V] = make_double2(a.a[2 * k], a.a[2 * k + 1]);
+}
+
+}  // namespace qcb

@@ -1,0 +1,231 @@
+// measure_f64.hip -- fused clover / plaquette / Q / E measurement (FP64).
+//
+// Replaces mean_plaquette_local (src/observables/observables_mpi.cpp:11-35),
+// clover_site (:66-90), local_q_e_clover (:93-112) and topo_q_e_clover[_global]
+// (:115-156) with one pass over the links.  The first clover leaf of plane
+// (mu,nu) is exactly the plaquette product of mean_plaquette_local, so the
+// plaquette comes for free.
+//
+// Work mapping: a workgroup is 64 consecutive sites x 3 plane pairs.  Pair 0
+// handles planes (0,1),(2,3), pair 1 (0,2),(1,3), pair 2 (0,3),(1,2), so each
+// thread owns one term of q = Re tr F01F23 - Re tr F02F13 + Re tr F03F12 and
+// two of the six energy terms and plaquettes.
+#include "kernels.hpp"
+#include "reduce.hpp"
+
+namespace qcb {
+
+// Clover field strength F = (C - C^dag)/8 (trace kept, as the reference does):
+// anti-hermitian, stored as the imaginary diagonal and the upper triangle.
+struct fclov {
+    double d[3];
+    cpx<double> o[3];  // (0,1), (0,2), (1,2)
+};
+
+// Sum of the four leaves of plane (MU,NU) at x, all anticlockwise, in the
+// order and association of clover_site (observables_mpi.cpp:78-86).  Adds
+// Re tr(leaf 0) to plaq.
+template <int MU, int NU>
+__device__ inline fclov clover64(const field64& f, unsigned s, const nbr& n, double& plaq) {
+    const unsigned xpmu = s + n.up[MU], xpnu = s + n.up[NU];
+    const unsigned xmmu = s + n.dn[MU], xmnu = s + n.dn[NU];
+    const unsigned xmmupnu = xmmu + n.up[NU], xmmumnu = xmmu + n.dn[NU];
+    const unsigned xpmumnu = xpmu + n.dn[NU];
+    mat3<double> C;
+    {  // U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
+        const mat3<double> a = mul(f.link(MU, s), f.link(NU, xpmu));
+        const mat3<double> b = mul_nd(a, f.link(MU, xpnu));
+        C = mul_nd(b, f.link(NU, s));
+        plaq += C.e[0][0].r + C.e[1][1].r + C.e[2][2].r;
+    }
+    {  // U_nu(x) U_mu(x-mu+nu)^dag U_nu(x-mu)^dag U_mu(x-mu)
+        const mat3<double> a = mul_nd(f.link(NU, s), f.link(MU, xmmupnu));
+        const mat3<double> b = mul_nd(a, f.link(NU, xmmu));
+        acc_mul(C, b, f.link(MU, xmmu));
+    }
+    {  // U_mu(x-mu)^dag U_nu(x-mu-nu)^dag U_mu(x-mu-nu) U_nu(x-nu)
+        const mat3<double> a = mul(f.link(NU, xmmumnu), f.link(MU, xmmu));
+        const mat3<double> b = mul(f.link(MU, xmmumnu), f.link(NU, xmnu));
+        acc_mul_dn(C, a, b);
+    }
+    {  // U_nu(x-nu)^dag U_mu(x-nu) U_nu(x+mu-nu) U_mu(x)^dag
+        const mat3<double> a = mul_dn(f.link(NU, xmnu), f.link(MU, xmnu));
+        const mat3<double> b = mul(a, f.link(NU, xpmumnu));
+        acc_mul_nd(C, b, f.link(MU, s));
+    }
+    fclov F;
+#pragma unroll
+    for (int k = 0; k < 3; k++) F.d[k] = (C.e[k][k].i + C.e[k][k].i) * 0.125;
+    F.o[0] = 0.125 * (C.e[0][1] - conj(C.e[1][0]));
+    F.o[1] = 0.125 * (C.e[0][2] - conj(C.e[2][0]));
+    F.o[2] = 0.125 * (C.e[1][2] - conj(C.e[2][1]));
+    return F;
+}
+
+template <int MU, int NU>
+__device__ inline void plaq64(const field64& f, unsigned s, const nbr& n, double& plaq) {
+    const unsigned xpmu = s + n.up[MU], xpnu = s + n.up[NU];
+    const mat3<double> a = mul(f.link(MU, s), f.link(NU, xpmu));
+    const mat3<double> b = mul_nd(a, f.link(MU, xpnu));
+    plaq += re_tr_nd(b, f.link(NU, s));
+}
+
+// |F|_F^2 for anti-hermitian F
+__device__ inline double fnorm2(const fclov& F) {
+    return F.d[0] * F.d[0] + F.d[1] * F.d[1] + F.d[2] * F.d[2] +
+           2.0 * (norm2(F.o[0]) + norm2(F.o[1]) + norm2(F.o[2]));
+}
+// Re tr(A B) for anti-hermitian A, B
+__device__ inline double re_tr_ab(const fclov& A, const fclov& B) {
+    double s = A.d[0] * B.d[0] + A.d[1] * B.d[1] + A.d[2] * B.d[2];
+    double o = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) o += A.o[k].r * B.o[k].r + A.o[k].i * B.o[k].i;
+    return -(s + 2.0 * o);
+}
+
+template <int M1, int N1, int M2, int N2, int SIGN, bool CLOVER>
+__device__ inline void pair_body64(const field64& f, unsigned s, const nbr& n, double& vp,
+                                   double& ve, double& vq) {
+    if constexpr (CLOVER) {
+        const fclov A = clover64<M1, N1>(f, s, n, vp);
+        const fclov B = clover64<M2, N2>(f, s, n, vp);
+        ve += 0.5 * (fnorm2(A) + fnorm2(B));
+        vq += SIGN * re_tr_ab(A, B);
+    } else {
+        plaq64<M1, N1>(f, s, n, vp);
+        plaq64<M2, N2>(f, s, n, vp);
+    }
+}
+
+template <bool CLOVER>
+__global__ __launch_bounds__(192) void k_measure64(const double2* __restrict__ U, lattice g,
+                                                   double* __restrict__ partials) {
+    __shared__ double lds[3 * 3];
+    const unsigned s = blockIdx.x * kSitesPerBlock + threadIdx.x;
+    double v[3] = {0.0, 0.0, 0.0};
+    if (s < g.V) {
+        const nbr n = neighbours(g, s);
+        const field64 f{U, g.V};
+        switch (threadIdx.y) {
+            case 0: pair_body64<0, 1, 2, 3, +1, CLOVER>(f, s, n, v[0], v[1], v[2]); break;
+            case 1: pair_body64<0, 2, 1, 3, -1, CLOVER>(f, s, n, v[0], v[1], v[2]); break;
+            default: pair_body64<0, 3, 1, 2, +1, CLOVER>(f, s, n, v[0], v[1], v[2]); break;
+        }
+    }
+    block_sum<3>(v, lds, 3);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        partials[3 * blockIdx.x + 0] = v[0];
+        partials[3 * blockIdx.x + 1] = v[1];
+        partials[3 * blockIdx.x + 2] = v[2];
+    }
+}
+
+// Ordered reduction of the per-workgroup partials and the reference's
+// normalisations: P = sum Re tr / 3 / (6V) (observables_mpi.cpp:48,67),
+// E = sum e / V (:173-176), Q = sum q / (4 pi^2) (:132).
+__global__ __launch_bounds__(256) void k_reduce_measure(const double* __restrict__ partials,
+                                                        unsigned nblk, double invV,
+                                                        double* __restrict__ out) {
+    __shared__ double lds[4 * 3];
+    double v[3] = {0.0, 0.0, 0.0};
+    for (unsigned b = threadIdx.x; b < nblk; b += blockDim.x) {
+        v[0] += partials[3 * b + 0];
+        v[1] += partials[3 * b + 1];
+        v[2] += partials[3 * b + 2];
+    }
+    block_sum<3>(v, lds, 4);
+    if (threadIdx.x == 0) {
+        out[0] = v[0] / 3.0 / 6.0 * invV;
+        out[1] = v[1] * invV;
+        out[2] = v[2] * (1.0 / (4.0 * M_PI * M_PI));
+    }
+}
+
+hipError_t launch_measure_f64(const double2* U, const lattice& g, double* partials, double* out,
+                              bool clover, hipStream_t st) {
+    const unsigned nblk = measure_blocks(g);
+    const dim3 block(kSitesPerBlock, 3);
+    if (clover)
+        k_measure64<true><<<nblk, block, 0, st>>>(U, g, partials);
+    else
+        k_measure64<false><<<nblk, block, 0, st>>>(U, g, partials);
+    k_reduce_measure<<<1, 256, 0, st>>>(partials, nblk, 1.0 / (double)g.V, out);
+    return hipGetLastError();
+}
+
+// ---- unitarity check --------------------------------------------------------
+__global__ __launch_bounds__(256) void k_unitarity64(const double2* __restrict__ U, lattice g,
+                                                     double* __restrict__ partials) {
+    __shared__ double lds[4 * 2];
+    const unsigned s = blockIdx.x * kSitesPerBlock + threadIdx.x;
+    double um = 0.0, dm = 0.0;
+    if (s < g.V) {
+        const field64 f{U, g.V};
+        const mat3<double> u = f.link(threadIdx.y, s);
+        const mat3<double> p = mul_nd(u, u);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                const double re = p.e[i][j].r - (i == j ? 1.0 : 0.0);
+                um = fmax(um, fmax(fabs(re), fabs(p.e[i][j].i)));
+            }
+        const cpx<double> det =
+            u.e[0][0] * (u.e[1][1] * u.e[2][2] - u.e[1][2] * u.e[2][1]) -
+            u.e[0][1] * (u.e[1][0] * u.e[2][2] - u.e[1][2] * u.e[2][0]) +
+            u.e[0][2] * (u.e[1][0] * u.e[2][1] - u.e[1][1] * u.e[2][0]);
+        dm = sqrt((det.r - 1.0) * (det.r - 1.0) + det.i * det.i);
+        if (!(um == um)) um = INFINITY;  // NaN -> inf so that max() keeps it
+        if (!(dm == dm)) dm = INFINITY;
+    }
+    um = wave_max(um);
+    dm = wave_max(dm);
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+    if ((tid & 63) == 0) {
+        lds[(tid >> 6) * 2] = um;
+        lds[(tid >> 6) * 2 + 1] = dm;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int w = 1; w < 4; w++) {
+            um = fmax(um, lds[2 * w]);
+            dm = fmax(dm, lds[2 * w + 1]);
+        }
+        partials[2 * blockIdx.x] = um;
+        partials[2 * blockIdx.x + 1] = dm;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_reduce_max2(const double* __restrict__ partials,
+                                                     unsigned nblk, double* __restrict__ out) {
+    __shared__ double lds[4 * 2];
+    double a = 0.0, b = 0.0;
+    for (unsigned k = threadIdx.x; k < nblk; k += blockDim.x) {
+        a = fmax(a, partials[2 * k]);
+        b = fmax(b, partials[2 * k + 1]);
+    }
+    a = wave_max(a);
+    b = wave_max(b);
+    if ((threadIdx.x & 63) == 0) {
+        lds[(threadIdx.x >> 6) * 2] = a;
+        lds[(threadIdx.x >> 6) * 2 + 1] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; w++) {
+            a = fmax(a, lds[2 * w]);
+            b = fmax(b, lds[2 * w + 1]);
+        }
+        out[0] = a;
+        out[1] = b;
+    }
+}
+
+hipError_t launch_unitarity_f64(const double2* U, const lattice& g, double* partials,
+                                double* out, hipStream_t st) {
+    const unsigned nblk = measure_blocks(g);
+    k_unitarity64<<<nblk, dim3(kSitesPerBlock, 4), 0, st>>>(U, g, partials);
+    k_reduce_max2<<<1, 256, 0, st>>>(partials, nblk, out);
+    return hipGetLastError();
+}
+
+}  // namespace qcb

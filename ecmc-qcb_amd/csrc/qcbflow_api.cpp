@@ -1,0 +1,655 @@
+// qcbflow_api.cpp -- the C ABI of libqcbflow (include/qcbflow.h).
+//
+// Host orchestration of the flow: buffer ownership, RK3 sequencing
+// (GradientFlow::rk3_step, src/flow/gradient_flow.cpp:124-135), the
+// topo_charge_flowed cadence (src/observables/observables_mpi.cpp:159-186),
+// ILDG I/O (src/io/ildg.cpp:34-237) and error mapping.  No CPU arithmetic
+// path exists: every observable and every flowed link comes from the HIP
+// kernels.
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "ildg.hpp"
+#include "kernels.hpp"
+#include "qcbflow.h"
+
+using qcb::lattice;
+
+struct qcbf_event {
+    int cls;
+    hipEvent_t a, b;
+};
+
+struct qcbf_flow {
+    int dev = 0;
+    int precision = QCBF_FP64;
+    lattice g{};
+    hipStream_t stream = nullptr;
+    double2* U[2] = {nullptr, nullptr};  // ping-pong link fields (SoA)
+    int cur = 0;                         // U[cur] is the flowing field
+    double2* acc = nullptr;              // RK3 su(3) accumulator (8 reals / link)
+    double2* F[2] = {nullptr, nullptr};  // force_0, force_1 of the unfused API
+    double* partials = nullptr;          // per-workgroup partial sums
+    double* dres = nullptr;              // device results of measurements
+    size_t dres_cap = 0;                 // in doubles
+    int* dwarn = nullptr;                // exp t > 1 flag
+    std::string err;
+    // timing
+    bool timing = false;
+    std::vector<qcbf_event> events;
+    size_t ev_used = 0;
+    double tot_ms[QCBF_K_COUNT] = {0, 0, 0, 0};
+    long cnt[QCBF_K_COUNT] = {0, 0, 0, 0};
+};
+
+static thread_local std::string g_err;
+
+static int fail(qcbf_flow* h, int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (h)
+        h->err = buf;
+    else
+        g_err = buf;
+    return code;
+}
+
+#define HIPCHK(h, expr)                                                                     \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail((h), e_ == hipErrorOutOfMemory ? QCBF_ERR_OOM : QCBF_ERR_HIP,       \
+                        "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+    } while (0)
+
+#define CHECK_H(h)                                                    \
+    do {                                                              \
+        if (!(h)) return fail(nullptr, QCBF_ERR_ARG, "null handle"); \
+        HIPCHK((h), hipSetDevice((h)->dev));                          \
+    } while (0)
+
+// ---- timing -----------------------------------------------------------------
+static int timing_flush(qcbf_flow* h) {
+    if (h->ev_used == 0) return QCBF_OK;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (size_t k = 0; k < h->ev_used; k++) {
+        float ms = 0.f;
+        HIPCHK(h, hipEventElapsedTime(&ms, h->events[k].a, h->events[k].b));
+        h->tot_ms[h->events[k].cls] += ms;
+        h->cnt[h->events[k].cls] += 1;
+    }
+    h->ev_used = 0;
+    return QCBF_OK;
+}
+static int timing_begin(qcbf_flow* h, int cls, qcbf_event** ev) {
+    *ev = nullptr;
+    if (!h->timing) return QCBF_OK;
+    if (h->ev_used == h->events.size()) {
+        int rc = timing_flush(h);
+        if (rc) return rc;
+    }
+    qcbf_event* e = &h->events[h->ev_used++];
+    e->cls = cls;
+    HIPCHK(h, hipEventRecord(e->a, h->stream));
+    *ev = e;
+    return QCBF_OK;
+}
+static int timing_end(qcbf_flow* h, qcbf_event* ev) {
+    if (ev) HIPCHK(h, hipEventRecord(ev->b, h->stream));
+    return QCBF_OK;
+}
+
+// ---- lifetime -----------------------------------------------------------------
+static void release(qcbf_flow* h) {
+    (void)hipSetDevice(h->dev);
+    for (auto& e : h->events) {
+        if (e.a) (void)hipEventDestroy(e.a);
+        if (e.b) (void)hipEventDestroy(e.b);
+    }
+    (void)hipFree(h->U[0]);
+    (void)hipFree(h->U[1]);
+    (void)hipFree(h->acc);
+    (void)hipFree(h->F[0]);
+    (void)hipFree(h->F[1]);
+    (void)hipFree(h->partials);
+    (void)hipFree(h->dres);
+    (void)hipFree(h->dwarn);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+extern "C" const char* qcbf_last_error(const qcbf_flow* h) {
+    return h ? h->err.c_str() : g_err.c_str();
+}
+
+static size_t link_field_bytes(const lattice& g) { return (size_t)g.V * 36 * sizeof(double2); }
+
+static int ensure_results(qcbf_flow* h, size_t ndoubles) {
+    if (ndoubles <= h->dres_cap) return QCBF_OK;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    (void)hipFree(h->dres);
+    h->dres = nullptr;
+    h->dres_cap = 0;
+    HIPCHK(h, hipMalloc(&h->dres, ndoubles * sizeof(double)));
+    h->dres_cap = ndoubles;
+    return QCBF_OK;
+}
+
+extern "C" int qcbf_create(const int dims[4], int precision, qcbf_flow** out) {
+    if (!dims || !out) return fail(nullptr, QCBF_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (precision != QCBF_FP64)
+        return fail(nullptr, QCBF_ERR_ARG, "precision %d not supported by this build", precision);
+    unsigned long long V = 1;
+    for (int d = 0; d < 4; d++) {
+        if (dims[d] < 2 || dims[d] > 4096)
+            return fail(nullptr, QCBF_ERR_ARG, "extent L[%d] = %d out of range [2, 4096]", d, dims[d]);
+        V *= (unsigned long long)dims[d];
+    }
+    if (V >= (1ull << 28)) return fail(nullptr, QCBF_ERR_ARG, "volume %llu too large", V);
+    qcbf_flow* h = new qcbf_flow();
+    h->precision = precision;
+    h->g = qcb::make_lattice(dims);
+    hipError_t e = hipGetDevice(&h->dev);
+    if (e != hipSuccess) {
+        delete h;
+        return fail(nullptr, QCBF_ERR_HIP, "hipGetDevice: %s", hipGetErrorString(e));
+    }
+#define CREATE_CHK(expr)                                                                      \
+    do {                                                                                      \
+        hipError_t e2 = (expr);                                                               \
+        if (e2 != hipSuccess) {                                                               \
+            release(h);                                                                       \
+            return fail(nullptr, e2 == hipErrorOutOfMemory ? QCBF_ERR_OOM : QCBF_ERR_HIP,     \
+                        "%s: %s", #expr, hipGetErrorString(e2));                              \
+        }                                                                                     \
+    } while (0)
+    CREATE_CHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    CREATE_CHK(hipMalloc(&h->U[0], link_field_bytes(h->g)));
+    CREATE_CHK(hipMalloc(&h->U[1], link_field_bytes(h->g)));
+    CREATE_CHK(hipMalloc(&h->acc, (size_t)h->g.V * 16 * sizeof(double2)));
+    CREATE_CHK(hipMalloc(&h->partials, (size_t)qcb::measure_blocks(h->g) * 3 * sizeof(double)));
+    CREATE_CHK(hipMalloc(&h->dwarn, sizeof(int)));
+    CREATE_CHK(hipMemsetAsync(h->dwarn, 0, sizeof(int), h->stream));
+    CREATE_CHK(hipMalloc(&h->dres, 64 * sizeof(double)));
+    h->dres_cap = 64;
+    if (qcb::launch_cold_start_f64(h->U[0], h->g, h->stream) != hipSuccess) {
+        release(h);
+        return fail(nullptr, QCBF_ERR_HIP, "cold start launch failed");
+    }
+    CREATE_CHK(hipStreamSynchronize(h->stream));
+#undef CREATE_CHK
+    *out = h;
+    return QCBF_OK;
+}
+
+extern "C" int qcbf_destroy(qcbf_flow* h) {
+    if (!h) return QCBF_OK;
+    (void)hipSetDevice(h->dev);
+    (void)hipStreamSynchronize(h->stream);
+    release(h);
+    return QCBF_OK;
+}
+
+extern "C" int qcbf_get_dims(const qcbf_flow* h, int dims[4], int* precision) {
+    if (!h || !dims) return fail(nullptr, QCBF_ERR_ARG, "null argument");
+    for (int d = 0; d < 4; d++) dims[d] = h->g.L[d];
+    if (precision) *precision = h->precision;
+    return QCBF_OK;
+}
+
+// ---- links in / out ------------------------------------------------------------
+extern "C" int qcbf_set_links(qcbf_flow* h, const double* links) {
+    CHECK_H(h);
+    if (!links) return fail(h, QCBF_ERR_ARG, "null links");
+    double* staging = reinterpret_cast<double*>(h->U[h->cur ^ 1]);
+    HIPCHK(h, hipMemcpyAsync(staging, links, (size_t)h->g.V * 72 * sizeof(double),
+                             hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, qcb::launch_aos_to_soa_f64(staging, h->U[h->cur], h->g, 0, h->g.V, false, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return QCBF_OK;
+}
+
+extern "C" int qcbf_get_links(qcbf_flow* h, double* links) {
+    CHECK_H(h);
+    if (!links) return fail(h, QCBF_ERR_ARG, "null links");
+    double* staging = reinterpret_cast<double*>(h->U[h->cur ^ 1]);
+    HIPCHK(h, qcb::launch_soa_to_aos_f64(h->U[h->cur], staging, h->g, 0, h->g.V, false, h->stream));
+    HIPCHK(h, hipMemcpyAsync(links, staging, (size_t)h->g.V * 72 * sizeof(double),
+                             hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return QCBF_OK;
+}
+
+extern "C" int qcbf_hot_start(qcbf_flow* h, uint64_t seed) {
+    CHECK_H(h);
+    HIPCHK(h, qcb::launch_hot_start_f64(h->U[h->cur], h->g, seed, h->stream));
+    return QCBF_OK;
+}
+
+extern "C" int qcbf_cold_start(qcbf_flow* h) {
+    CHECK_H(h);
+    HIPCHK(h, qcb::launch_cold_start_f64(h->U[h->cur], h->g, h->stream));
+    return QCBF_OK;
+}
+
+// ---- ILDG ----------------------------------------------------------------------
+namespace {
+struct pinned_pair {
+    void* p[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    ~pinned_pair() {
+        for (int k = 0; k < 2; k++) {
+            if (ev[k]) {
+                (void)hipEventSynchronize(ev[k]);
+                (void)hipEventDestroy(ev[k]);
+            }
+            if (p[k]) (void)hipHostFree(p[k]);
+        }
+    }
+};
+struct file_closer {
+    FILE* fp;
+    ~file_closer() {
+        if (fp) fclose(fp);
+    }
+};
+constexpr size_t kIoChunk = 64ull << 20;
+}  // namespace
+
+extern "C" int qcbf_load_ildg(qcbf_flow* h, const char* path) {
+    CHECK_H(h);
+    if (!path) return fail(h, QCBF_ERR_ARG, "null path");
+    FILE* fp = fopen(path, "rb");
+    if (!fp) return fail(h, QCBF_ERR_IO, "cannot open '%s' for reading", path);
+    file_closer fc{fp};
+    qcb::ildg_info info;
+    std::string msg;
+    int rc = qcb::ildg_scan(fp, &info, &msg);
+    if (rc) return fail(h, rc, "%s: %s", path, msg.c_str());
+    if (info.has_format) {
+        for (int d = 0; d < 4; d++)
+            if (info.dims[d] != h->g.L[d])
+                return fail(h, QCBF_ERR_FORMAT,
+                            "%s: ildg-format extents %dx%dx%dx%d differ from the handle's %dx%dx%dx%d",
+                            path, info.dims[0], info.dims[1], info.dims[2], info.dims[3], h->g.L[0],
+                            h->g.L[1], h->g.L[2], h->g.L[3]);
+        if (info.precision != 64 && info.precision != 32)
+            return fail(h, QCBF_ERR_FORMAT, "%s: unsupported precision %d", path, info.precision);
+    }
+    const size_t word = info.precision == 32 ? 4 : 8;
+    const unsigned long long expect = (unsigned long long)h->g.V * 72 * word;
+    if (info.data_bytes != expect)
+        return fail(h, QCBF_ERR_FORMAT, "%s: binary size %llu bytes, expected %llu", path,
+                    info.data_bytes, expect);
+    // Stream the payload through two pinned buffers into the staging field.
+    char* staging = reinterpret_cast<char*>(h->U[h->cur ^ 1]);
+    pinned_pair pp;
+    for (int k = 0; k < 2; k++) {
+        HIPCHK(h, hipHostMalloc(&pp.p[k], kIoChunk, hipHostMallocDefault));
+        HIPCHK(h, hipEventCreateWithFlags(&pp.ev[k], hipEventDisableTiming));
+    }
+    if (fseeko(fp, info.data_offset, SEEK_SET) != 0)
+        return fail(h, QCBF_ERR_IO, "%s: seek failed", path);
+    size_t done = 0;
+    int k = 0;
+    bool used[2] = {false, false};
+    while (done < expect) {
+        const size_t n = (expect - done < kIoChunk) ? (size_t)(expect - done) : kIoChunk;
+        if (used[k]) HIPCHK(h, hipEventSynchronize(pp.ev[k]));
+        if (fread(pp.p[k], 1, n, fp) != n) return fail(h, QCBF_ERR_IO, "%s: short read", path);
+        HIPCHK(h, hipMemcpyAsync(staging + done, pp.p[k], n, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipEventRecord(pp.ev[k], h->stream));
+        used[k] = true;
+        done += n;
+        k ^= 1;
+    }
+    if (word == 8)
+        HIPCHK(h, qcb::launch_aos_to_soa_f64(reinterpret_cast<double*>(staging), h->U[h->cur], h->g,
+                                             0, h->g.V, true, h->stream));
+    else
+        HIPCHK(h, qcb::launch_aos32_to_soa_f64(reinterpret_cast<float*>(staging), h->U[h->cur],
+                                               h->g, 0, h->g.V, true, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return QCBF_OK;
+}
+
+extern "C" int qcbf_save_ildg(qcbf_flow* h, const char* path) {
+    CHECK_H(h);
+    if (!path) return fail(h, QCBF_ERR_ARG, "null path");
+    double* staging = reinterpret_cast<double*>(h->U[h->cur ^ 1]);
+    HIPCHK(h, qcb::launch_soa_to_aos_f64(h->U[h->cur], staging, h->g, 0, h->g.V, true, h->stream));
+    FILE* fp = fopen(path, "wb");
+    if (!fp) return fail(h, QCBF_ERR_IO, "cannot open '%s' for writing", path);
+    file_closer fc{fp};
+    const std::string xml = qcb::ildg_format_xml(h->g.L, 64);
+    const uint64_t nbin = (uint64_t)h->g.V * 72 * sizeof(double);
+    bool ok = qcb::lime_write_header(fp, "ildg-format", xml.size(), true, true);
+    ok = ok && fwrite(xml.data(), 1, xml.size(), fp) == xml.size();
+    static const char zeros[8] = {0};
+    const size_t pad = (8 - xml.size() % 8) % 8;
+    ok = ok && fwrite(zeros, 1, pad, fp) == pad;
+    ok = ok && qcb::lime_write_header(fp, "ildg-binary-data", nbin, false, true);
+    if (!ok) return fail(h, QCBF_ERR_IO, "%s: short write of LIME headers", path);
+    pinned_pair pp;
+    for (int k = 0; k < 2; k++) HIPCHK(h, hipHostMalloc(&pp.p[k], kIoChunk, hipHostMallocDefault));
+    const char* src = reinterpret_cast<const char*>(staging);
+    size_t done = 0;
+    while (done < nbin) {
+        const size_t n = (nbin - done < kIoChunk) ? (size_t)(nbin - done) : kIoChunk;
+        HIPCHK(h, hipMemcpyAsync(pp.p[0], src + done, n, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        if (fwrite(pp.p[0], 1, n, fp) != n) return fail(h, QCBF_ERR_IO, "%s: short write", path);
+        done += n;
+    }
+    if (fflush(fp) != 0) return fail(h, QCBF_ERR_IO, "%s: flush failed", path);
+    return QCBF_OK;
+}
+
+// ---- flow ------------------------------------------------------------------------
+static int rk3_once(qcbf_flow* h, double eps) {
+    const double cz[3] = {0.25 * eps, (8.0 / 9.0) * eps, 0.75 * eps};
+    const double ca[3] = {0.0, (17.0 / 36.0) * eps, eps};
+    for (int stage = 1; stage <= 3; stage++) {
+        qcbf_event* ev;
+        int rc = timing_begin(h, QCBF_K_STAGE1 + stage - 1, &ev);
+        if (rc) return rc;
+        HIPCHK(h, qcb::launch_rk3_stage_f64(stage, h->U[h->cur], h->U[h->cur ^ 1], h->acc, h->g,
+                                            cz[stage - 1], ca[stage - 1], h->dwarn, h->stream));
+        rc = timing_end(h, ev);
+        if (rc) return rc;
+        h->cur ^= 1;
+    }
+    return QCBF_OK;
+}
+
+extern "C" int qcbf_rk3_step(qcbf_flow* h, double eps, int n_steps) {
+    CHECK_H(h);
+    if (n_steps < 0) return fail(h, QCBF_ERR_ARG, "n_steps < 0");
+    for (int k = 0; k < n_steps; k++) {
+        int rc = rk3_once(h, eps);
+        if (rc) return rc;
+    }
+    return QCBF_OK;
+}
+
+static int ensure_force(qcbf_flow* h, int i) {
+    if (!h->F[i]) HIPCHK(h, hipMalloc(&h->F[i], (size_t)h->g.V * 16 * sizeof(double2)));
+    return QCBF_OK;
+}
+
+extern "C" int qcbf_compute_force(qcbf_flow* h, int i) {
+    CHECK_H(h);
+    if (i != 0 && i != 1) return fail(h, QCBF_ERR_ARG, "force index %d not 0 or 1", i);
+    int rc = ensure_force(h, i);
+    if (rc) return rc;
+    HIPCHK(h, qcb::launch_force_f64(h->U[h->cur], h->F[i], h->g, h->stream));
+    return QCBF_OK;
+}
+
+static int update(qcbf_flow* h, int which, double eps) {
+    if (!h->F[0] || (which > 1 && !h->F[1]))
+        return fail(h, QCBF_ERR_STATE, "compute_w%d needs the forces it combines", which);
+    HIPCHK(h, qcb::launch_update_f64(which, h->U[h->cur], h->F[0], h->F[1], h->g, eps, h->dwarn,
+                                     h->stream));
+    return QCBF_OK;
+}
+extern "C" int qcbf_compute_w1(qcbf_flow* h, double eps) {
+    CHECK_H(h);
+    return update(h, 1, eps);
+}
+extern "C" int qcbf_compute_w2(qcbf_flow* h, double eps) {
+    CHECK_H(h);
+    return update(h, 2, eps);
+}
+extern "C" int qcbf_compute_w3(qcbf_flow* h, double eps) {
+    CHECK_H(h);
+    return update(h, 3, eps);
+}
+extern "C" int qcbf_replace_force_0(qcbf_flow* h) {
+    CHECK_H(h);
+    if (!h->F[0] || !h->F[1]) return fail(h, QCBF_ERR_STATE, "replace_force_0 needs both forces");
+    HIPCHK(h, qcb::launch_replace_force0_f64(h->F[0], h->F[1], h->g, h->stream));
+    return QCBF_OK;
+}
+extern "C" int qcbf_get_force(qcbf_flow* h, int i, double* out) {
+    CHECK_H(h);
+    if ((i != 0 && i != 1) || !out) return fail(h, QCBF_ERR_ARG, "bad force index or null output");
+    if (!h->F[i]) return fail(h, QCBF_ERR_STATE, "force_%d has not been computed", i);
+    double* staging = reinterpret_cast<double*>(h->U[h->cur ^ 1]);
+    HIPCHK(h, qcb::launch_alg_to_aos_f64(h->F[i], staging, h->g, h->stream));
+    HIPCHK(h, hipMemcpyAsync(out, staging, (size_t)h->g.V * 72 * sizeof(double),
+                             hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return QCBF_OK;
+}
+
+// ---- observables ---------------------------------------------------------------
+static int measure_into(qcbf_flow* h, double* dout, bool clover) {
+    qcbf_event* ev;
+    int rc = timing_begin(h, QCBF_K_MEASURE, &ev);
+    if (rc) return rc;
+    HIPCHK(h, qcb::launch_measure_f64(h->U[h->cur], h->g, h->partials, dout, clover, h->stream));
+    return timing_end(h, ev);
+}
+
+static int measure_now(qcbf_flow* h, bool clover, double out[3]) {
+    int rc = measure_into(h, h->dres, clover);
+    if (rc) return rc;
+    HIPCHK(h, hipMemcpyAsync(out, h->dres, 3 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return QCBF_OK;
+}
+
+extern "C" int qcbf_plaquette(qcbf_flow* h, double* plaq) {
+    CHECK_H(h);
+    if (!plaq) return fail(h, QCBF_ERR_ARG, "null output");
+    double r[3];
+    int rc = measure_now(h, false, r);
+    if (rc) return rc;
+    *plaq = r[0];
+    return QCBF_OK;
+}
+
+extern "C" int qcbf_q_e_clover(qcbf_flow* h, double* Q, double* E) {
+    CHECK_H(h);
+    if (!Q || !E) return fail(h, QCBF_ERR_ARG, "null output");
+    double r[3];
+    int rc = measure_now(h, true, r);
+    if (rc) return rc;
+    *E = r[1];
+    *Q = r[2];
+    return QCBF_OK;
+}
+
+extern "C" int qcbf_measure(qcbf_flow* h, double* plaq, double* E, double* Q) {
+    CHECK_H(h);
+    if (!plaq || !Q || !E) return fail(h, QCBF_ERR_ARG, "null output");
+    double r[3];
+    int rc = measure_now(h, true, r);
+    if (rc) return rc;
+    *plaq = r[0];
+    *E = r[1];
+    *Q = r[2];
+    return QCBF_OK;
+}
+
+extern "C" int qcbf_flow_measure(qcbf_flow* h, double eps, int n_steps_gf, int n_rk_steps,
+                                 double* tQE, double* plaq) {
+    CHECK_H(h);
+    if (n_steps_gf < 0 || n_rk_steps < 0 || (n_steps_gf > 0 && !tQE))
+        return fail(h, QCBF_ERR_ARG, "bad flow_measure arguments");
+    if (n_steps_gf == 0) return QCBF_OK;
+    int rc = ensure_results(h, (size_t)n_steps_gf * 3);
+    if (rc) return rc;
+    std::vector<double> tvals(n_steps_gf);
+    double t = 0.0;
+    for (int n = 0; n < n_steps_gf; n++) {
+        tvals[n] = t;
+        rc = measure_into(h, h->dres + 3 * (size_t)n, true);
+        if (rc) return rc;
+        if (n != n_steps_gf - 1) {
+            for (int k = 0; k < n_rk_steps; k++) {
+                rc = rk3_once(h, eps);
+                if (rc) return rc;
+                t += eps;
+            }
+        }
+    }
+    std::vector<double> r((size_t)n_steps_gf * 3);
+    HIPCHK(h, hipMemcpyAsync(r.data(), h->dres, r.size() * sizeof(double), hipMemcpyDeviceToHost,
+                             h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (int n = 0; n < n_steps_gf; n++) {
+        tQE[3 * n] = tvals[n];
+        tQE[3 * n + 1] = r[3 * n + 2];
+        tQE[3 * n + 2] = r[3 * n + 1];
+        if (plaq) plaq[n] = r[3 * n];
+    }
+    return QCBF_OK;
+}
+
+extern "C" int qcbf_exp_warnings(qcbf_flow* h, int* flagged) {
+    CHECK_H(h);
+    if (!flagged) return fail(h, QCBF_ERR_ARG, "null output");
+    int v = 0;
+    HIPCHK(h, hipMemcpyAsync(&v, h->dwarn, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemsetAsync(h->dwarn, 0, sizeof(int), h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    *flagged = v;
+    return QCBF_OK;
+}
+
+extern "C" int qcbf_unitarity(qcbf_flow* h, double* max_dev, double* max_det_dev) {
+    CHECK_H(h);
+    if (!max_dev || !max_det_dev) return fail(h, QCBF_ERR_ARG, "null output");
+    HIPCHK(h, qcb::launch_unitarity_f64(h->U[h->cur], h->g, h->partials, h->dres, h->stream));
+    double r[2];
+    HIPCHK(h, hipMemcpyAsync(r, h->dres, sizeof(r), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    *max_dev = r[0];
+    *max_det_dev = r[1];
+    return QCBF_OK;
+}
+
+extern "C" int qcbf_synchronize(qcbf_flow* h) {
+    CHECK_H(h);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return QCBF_OK;
+}
+
+// ---- exp on independent matrices -------------------------------------------------
+extern "C" int qcbf_su3_exp(const double* Z, int n, double coeff, double* out, int* flagged) {
+    if (n < 0 || (n > 0 && (!Z || !out))) return fail(nullptr, QCBF_ERR_ARG, "bad arguments");
+    if (flagged) *flagged = 0;
+    if (n == 0) return QCBF_OK;
+    const size_t bytes = (size_t)n * 18 * sizeof(double);
+    double *dz = nullptr, *dout = nullptr;
+    int* dw = nullptr;
+    int rc = QCBF_OK;
+    hipError_t e;
+    auto chk = [&](hipError_t x, const char* what) {
+        if (x != hipSuccess && rc == QCBF_OK)
+            rc = fail(nullptr, x == hipErrorOutOfMemory ? QCBF_ERR_OOM : QCBF_ERR_HIP, "%s: %s",
+                      what, hipGetErrorString(x));
+        return x == hipSuccess;
+    };
+    if (chk(hipMalloc(&dz, bytes), "hipMalloc") && chk(hipMalloc(&dout, bytes), "hipMalloc") &&
+        chk(hipMalloc(&dw, sizeof(int)), "hipMalloc") && chk(hipMemset(dw, 0, sizeof(int)), "hipMemset") &&
+        chk(hipMemcpy(dz, Z, bytes, hipMemcpyHostToDevice), "hipMemcpy") &&
+        chk(qcb::launch_exp_batch_f64(dz, dout, n, coeff, dw, nullptr), "exp kernel") &&
+        chk(hipMemcpy(out, dout, bytes, hipMemcpyDeviceToHost), "hipMemcpy")) {
+        int w = 0;
+        e = hipMemcpy(&w, dw, sizeof(int), hipMemcpyDeviceToHost);
+        if (chk(e, "hipMemcpy") && flagged) *flagged = w;
+    }
+    (void)hipFree(dz);
+    (void)hipFree(dout);
+    (void)hipFree(dw);
+    return rc;
+}
+
+// ---- measurement support ------------------------------------------------------------
+extern "C" int qcbf_timing_enable(qcbf_flow* h, int on) {
+    CHECK_H(h);
+    int rc = timing_flush(h);
+    if (rc) return rc;
+    if (on && h->events.empty()) {
+        h->events.resize(4096);
+        for (auto& ev : h->events) ev.a = ev.b = nullptr;
+        for (auto& ev : h->events) {
+            HIPCHK(h, hipEventCreate(&ev.a));
+            HIPCHK(h, hipEventCreate(&ev.b));
+        }
+    }
+    h->timing = on != 0;
+    for (int k = 0; k < QCBF_K_COUNT; k++) {
+        h->tot_ms[k] = 0.0;
+        h->cnt[k] = 0;
+    }
+    return QCBF_OK;
+}
+
+extern "C" int qcbf_timing_read(qcbf_flow* h, double total_ms[QCBF_K_COUNT],
+                                long launches[QCBF_K_COUNT]) {
+    CHECK_H(h);
+    int rc = timing_flush(h);
+    if (rc) return rc;
+    for (int k = 0; k < QCBF_K_COUNT; k++) {
+        if (total_ms) total_ms[k] = h->tot_ms[k];
+        if (launches) launches[k] = h->cnt[k];
+    }
+    return QCBF_OK;
+}
+
+extern "C" int qcbf_copy_bench(qcbf_flow* h, size_t bytes, int reps, double* ms) {
+    CHECK_H(h);
+    if (!ms || reps <= 0 || bytes < 16) return fail(h, QCBF_ERR_ARG, "bad copy_bench arguments");
+    const size_t n = bytes / sizeof(double2);
+    double2 *a = nullptr, *b = nullptr;
+    HIPCHK(h, hipMalloc(&a, n * sizeof(double2)));
+    hipError_t e = hipMalloc(&b, n * sizeof(double2));
+    if (e != hipSuccess) {
+        (void)hipFree(a);
+        return fail(h, QCBF_ERR_OOM, "hipMalloc: %s", hipGetErrorString(e));
+    }
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipMemsetAsync(a, 0, n * sizeof(double2), h->stream);
+    (void)qcb::launch_copy(a, b, n, h->stream);
+    (void)hipEventRecord(e0, h->stream);
+    for (int r = 0; r < reps; r++) (void)qcb::launch_copy(a, b, n, h->stream);
+    (void)hipEventRecord(e1, h->stream);
+    e = hipEventSynchronize(e1);
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess) return fail(h, QCBF_ERR_HIP, "copy bench: %s", hipGetErrorString(e));
+    *ms = (double)t / reps;
+    return QCBF_OK;
+}
+
+extern "C" int qcbf_device_info(char* name, int name_len, int* compute_units) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(nullptr, QCBF_ERR_HIP, "hipGetDevice: %s", hipGetErrorString(e));
+    hipDeviceProp_t p;
+    e = hipGetDeviceProperties(&p, dev);
+    if (e != hipSuccess)
+        return fail(nullptr, QCBF_ERR_HIP, "hipGetDeviceProperties: %s", hipGetErrorString(e));
+    if (name && name_len > 0) snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
+    if (compute_units) *compute_units = p.multiProcessorCount;
+    return QCBF_OK;
+}

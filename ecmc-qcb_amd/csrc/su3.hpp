@@ -1,0 +1,307 @@
+// su3.hpp -- device-inline 3x3 complex algebra for SU(3) links and su(3) elements.
+//
+// Replaces the Eigen fixed-size products the reference uses for every 3x3
+// operation on the flow path (src/su3/utils.h:12, src/gauge/GaugeField.h:16).
+// Everything is templated on the real type so the FP64 path and the FP32-mixed
+// path share one implementation; all of it runs on the vector ALUs (3x3
+// complex products are not a dense contraction worth an MFMA tile).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace qcb {
+
+template <typename T>
+struct cpx {
+    T r, i;
+};
+
+template <typename T>
+__host__ __device__ inline cpx<T> mk(T r, T i) {
+    return cpx<T>{r, i};
+}
+template <typename T>
+__host__ __device__ inline cpx<T> operator+(cpx<T> a, cpx<T> b) {
+    return {a.r + b.r, a.i + b.i};
+}
+template <typename T>
+__host__ __device__ inline cpx<T> operator-(cpx<T> a, cpx<T> b) {
+    return {a.r - b.r, a.i - b.i};
+}
+template <typename T>
+__host__ __device__ inline cpx<T> operator*(cpx<T> a, cpx<T> b) {
+    return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r};
+}
+template <typename T>
+__host__ __device__ inline cpx<T> operator*(T s, cpx<T> a) {
+    return {s * a.r, s * a.i};
+}
+template <typename T>
+__host__ __device__ inline cpx<T> conj(cpx<T> a) {
+    return {a.r, -a.i};
+}
+// acc + a*b
+template <typename T>
+__host__ __device__ inline cpx<T> madd(cpx<T> acc, cpx<T> a, cpx<T> b) {
+    return {acc.r + a.r * b.r - a.i * b.i, acc.i + a.r * b.i + a.i * b.r};
+}
+// acc + a*conj(b)
+template <typename T>
+__host__ __device__ inline cpx<T> madd_c(cpx<T> acc, cpx<T> a, cpx<T> b) {
+    return {acc.r + a.r * b.r + a.i * b.i, acc.i + a.i * b.r - a.r * b.i};
+}
+// acc + conj(a)*b
+template <typename T>
+__host__ __device__ inline cpx<T> madd_cl(cpx<T> acc, cpx<T> a, cpx<T> b) {
+    return {acc.r + a.r * b.r + a.i * b.i, acc.i + a.r * b.i - a.i * b.r};
+}
+template <typename T>
+__host__ __device__ inline T norm2(cpx<T> a) {
+    return a.r * a.r + a.i * a.i;
+}
+
+// Row-major 3x3 complex matrix.
+template <typename T>
+struct mat3 {
+    cpx<T> e[3][3];
+};
+
+// Traceless anti-hermitian 3x3 matrix (an element of su(3)) in 8 reals:
+//   [[ i d0,   p,      q    ],
+//    [ -p*,    i d1,   r    ],
+//    [ -q*,    -r*,    i d2 ]],  d2 = -d0 - d1.
+// a[0]=d0 a[1]=d1 a[2..3]=p a[4..5]=q a[6..7]=r.
+template <typename T>
+struct alg8 {
+    T a[8];
+};
+
+template <typename T>
+__host__ __device__ inline mat3<T> zero3() {
+    mat3<T> m;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) m.e[i][j] = {T(0), T(0)};
+    return m;
+}
+
+// R = A B
+template <typename T>
+__host__ __device__ inline mat3<T> mul(const mat3<T>& A, const mat3<T>& B) {
+    mat3<T> R;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            cpx<T> s = A.e[i][0] * B.e[0][j];
+            s = madd(s, A.e[i][1], B.e[1][j]);
+            s = madd(s, A.e[i][2], B.e[2][j]);
+            R.e[i][j] = s;
+        }
+    return R;
+}
+// R = A B^dag
+template <typename T>
+__host__ __device__ inline mat3<T> mul_nd(const mat3<T>& A, const mat3<T>& B) {
+    mat3<T> R;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            cpx<T> s = A.e[i][0] * conj(B.e[j][0]);
+            s = madd_c(s, A.e[i][1], B.e[j][1]);
+            s = madd_c(s, A.e[i][2], B.e[j][2]);
+            R.e[i][j] = s;
+        }
+    return R;
+}
+// R = A^dag B
+template <typename T>
+__host__ __device__ inline mat3<T> mul_dn(const mat3<T>& A, const mat3<T>& B) {
+    mat3<T> R;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            cpx<T> s = conj(A.e[0][i]) * B.e[0][j];
+            s = madd_cl(s, A.e[1][i], B.e[1][j]);
+            s = madd_cl(s, A.e[2][i], B.e[2][j]);
+            R.e[i][j] = s;
+        }
+    return R;
+}
+// R += A B
+template <typename T>
+__host__ __device__ inline void acc_mul(mat3<T>& R, const mat3<T>& A, const mat3<T>& B) {
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            cpx<T> s = R.e[i][j];
+            s = madd(s, A.e[i][0], B.e[0][j]);
+            s = madd(s, A.e[i][1], B.e[1][j]);
+            s = madd(s, A.e[i][2], B.e[2][j]);
+            R.e[i][j] = s;
+        }
+}
+// R += A B^dag
+template <typename T>
+__host__ __device__ inline void acc_mul_nd(mat3<T>& R, const mat3<T>& A, const mat3<T>& B) {
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            cpx<T> s = R.e[i][j];
+            s = madd_c(s, A.e[i][0], B.e[j][0]);
+            s = madd_c(s, A.e[i][1], B.e[j][1]);
+            s = madd_c(s, A.e[i][2], B.e[j][2]);
+            R.e[i][j] = s;
+        }
+}
+// R += A^dag B
+template <typename T>
+__host__ __device__ inline void acc_mul_dn(mat3<T>& R, const mat3<T>& A, const mat3<T>& B) {
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            cpx<T> s = R.e[i][j];
+            s = madd_cl(s, A.e[0][i], B.e[0][j]);
+            s = madd_cl(s, A.e[1][i], B.e[1][j]);
+            s = madd_cl(s, A.e[2][i], B.e[2][j]);
+            R.e[i][j] = s;
+        }
+}
+// Re tr(A B^dag) = sum_ij Re(A_ij conj(B_ij))
+template <typename T>
+__host__ __device__ inline T re_tr_nd(const mat3<T>& A, const mat3<T>& B) {
+    T s = T(0);
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) s += A.e[i][j].r * B.e[i][j].r + A.e[i][j].i * B.e[i][j].i;
+    return s;
+}
+
+// -P_TA(M) with P_TA(M) = (M - M^dag)/2 - tr((M - M^dag)/2)/3 * 1
+// (proj_lie_su3, src/su3/utils.cpp:130-134), returned in 8 reals.
+template <typename T>
+__host__ __device__ inline alg8<T> neg_proj_ta(const mat3<T>& M) {
+    alg8<T> z;
+    const T h = T(0.5);
+    T tr3 = (M.e[0][0].i + M.e[1][1].i + M.e[2][2].i) / T(3);
+    z.a[0] = -(M.e[0][0].i - tr3);
+    z.a[1] = -(M.e[1][1].i - tr3);
+    z.a[2] = -h * (M.e[0][1].r - M.e[1][0].r);
+    z.a[3] = -h * (M.e[0][1].i + M.e[1][0].i);
+    z.a[4] = -h * (M.e[0][2].r - M.e[2][0].r);
+    z.a[5] = -h * (M.e[0][2].i + M.e[2][0].i);
+    z.a[6] = -h * (M.e[1][2].r - M.e[2][1].r);
+    z.a[7] = -h * (M.e[1][2].i + M.e[2][1].i);
+    return z;
+}
+
+template <typename T>
+__host__ __device__ inline mat3<T> alg_to_mat(const alg8<T>& x) {
+    mat3<T> m;
+    T d2 = -x.a[0] - x.a[1];
+    m.e[0][0] = {T(0), x.a[0]};
+    m.e[1][1] = {T(0), x.a[1]};
+    m.e[2][2] = {T(0), d2};
+    m.e[0][1] = {x.a[2], x.a[3]};
+    m.e[0][2] = {x.a[4], x.a[5]};
+    m.e[1][2] = {x.a[6], x.a[7]};
+    m.e[1][0] = {-x.a[2], x.a[3]};
+    m.e[2][0] = {-x.a[4], x.a[5]};
+    m.e[2][1] = {-x.a[6], x.a[7]};
+    return m;
+}
+
+// Taylor coefficients c_n = 1/n!, n = 0..18, formed by repeated division
+// exactly as exp_su3_luscher does (src/su3/utils.cpp:162-166).
+struct horner_coeffs {
+    double c[19];
+};
+__host__ __device__ constexpr horner_coeffs make_horner_coeffs() {
+    horner_coeffs h{};
+    h.c[0] = 1.0;
+    for (int n = 1; n <= 18; n++) h.c[n] = h.c[n - 1] / static_cast<double>(n);
+    return h;
+}
+
+// Cayley-Hamilton coefficients of exp(X) = q0 1 + q1 X + q2 X^2 for X in su(3)
+// with t = -1/2 tr X^2 and d = Re(i det X), from the fixed 18-step Horner
+// recursion of exp_su3_luscher (src/su3/utils.cpp:168-186):
+//   q <- (c_n - i d q2, q0 - t q2, q1),  n = 17..0.
+// Evaluated in the type H (double in every path).
+template <typename H>
+__host__ __device__ inline void horner18(H t, H d, cpx<H>& q0, cpx<H>& q1, cpx<H>& q2) {
+    constexpr horner_coeffs hc = make_horner_coeffs();
+    cpx<H> a0 = {H(hc.c[18]), H(0)}, a1 = {H(0), H(0)}, a2 = {H(0), H(0)};
+#pragma unroll
+    for (int n = 17; n >= 0; n--) {
+        cpx<H> n0 = {H(hc.c[n]) + d * a2.i, -d * a2.r};  // c_n - i d q2
+        cpx<H> n1 = {a0.r - t * a2.r, a0.i - t * a2.i};   // q0 - t q2
+        a2 = a1;
+        a1 = n1;
+        a0 = n0;
+    }
+    q0 = a0;
+    q1 = a1;
+    q2 = a2;
+}
+
+// exp(X) for X in su(3), following exp_su3_luscher (src/su3/utils.cpp:145-193).
+// t and d use the closed forms for a traceless anti-hermitian X:
+//   t = (d0^2 + d1^2 + d2^2)/2 + |p|^2 + |q|^2 + |r|^2
+//   d = d0 d1 d2 - d0 |r|^2 - d1 |q|^2 - d2 |p|^2 + 2 Im(p r q*)
+// Sets *warn when t > 1 (the reference prints a warning, utils.cpp:153-155).
+// X^2 is hermitian, so only its diagonal and upper triangle are formed.
+template <typename T, typename H>
+__host__ __device__ inline mat3<T> exp_su3(const alg8<T>& x, bool* warn) {
+    const T d0 = x.a[0], d1 = x.a[1], d2 = -x.a[0] - x.a[1];
+    const cpx<T> p = {x.a[2], x.a[3]}, q = {x.a[4], x.a[5]}, r = {x.a[6], x.a[7]};
+    const T pp = norm2(p), qq = norm2(q), rr = norm2(r);
+    // X^2 (hermitian): diagonal real, upper triangle
+    const T s00 = -(d0 * d0) - pp - qq;
+    const T s11 = -pp - d1 * d1 - rr;
+    const T s22 = -qq - rr - d2 * d2;
+    // (X^2)01 = i(d0+d1) p - q r*
+    const cpx<T> s01 = {-(d0 + d1) * p.i - (q.r * r.r + q.i * r.i),
+                        (d0 + d1) * p.r - (q.i * r.r - q.r * r.i)};
+    // (X^2)02 = i(d0+d2) q + p r
+    const cpx<T> s02 = {-(d0 + d2) * q.i + (p.r * r.r - p.i * r.i),
+                        (d0 + d2) * q.r + (p.r * r.i + p.i * r.r)};
+    // (X^2)12 = -p* q + i(d1+d2) r
+    const cpx<T> s12 = {-(p.r * q.r + p.i * q.i) - (d1 + d2) * r.i,
+                        -(p.r * q.i - p.i * q.r) + (d1 + d2) * r.r};
+
+    H hd0 = H(d0), hd1 = H(d1), hd2 = H(d2);
+    H tt = H(0.5) * (hd0 * hd0 + hd1 * hd1 + hd2 * hd2) + H(pp) + H(qq) + H(rr);
+    // Im(p r q*) = Im(p r) Re q - Re(p r) Im q
+    H prr = H(p.r) * H(r.r) - H(p.i) * H(r.i);
+    H pri = H(p.r) * H(r.i) + H(p.i) * H(r.r);
+    H dd = hd0 * hd1 * hd2 - hd0 * H(rr) - hd1 * H(qq) - hd2 * H(pp) +
+           H(2) * (pri * H(q.r) - prr * H(q.i));
+    if (warn && tt > H(1)) *warn = true;
+    cpx<H> h0, h1, h2;
+    horner18<H>(tt, dd, h0, h1, h2);
+    const cpx<T> q0 = {T(h0.r), T(h0.i)}, q1 = {T(h1.r), T(h1.i)}, q2 = {T(h2.r), T(h2.i)};
+
+    mat3<T> E;
+    // diagonal: q0 + q1 (i d_k) + q2 s_kk
+    E.e[0][0] = {q0.r - q1.i * d0 + q2.r * s00, q0.i + q1.r * d0 + q2.i * s00};
+    E.e[1][1] = {q0.r - q1.i * d1 + q2.r * s11, q0.i + q1.r * d1 + q2.i * s11};
+    E.e[2][2] = {q0.r - q1.i * d2 + q2.r * s22, q0.i + q1.r * d2 + q2.i * s22};
+    // upper: q1 X_ij + q2 S_ij ; lower: q1 X_ji + q2 conj(S_ij), X_ji = -conj(X_ij)
+    const cpx<T> mp = {-p.r, p.i}, mq = {-q.r, q.i}, mr = {-r.r, r.i};
+    E.e[0][1] = madd(q1 * p, q2, s01);
+    E.e[0][2] = madd(q1 * q, q2, s02);
+    E.e[1][2] = madd(q1 * r, q2, s12);
+    E.e[1][0] = madd(q1 * mp, q2, conj(s01));
+    E.e[2][0] = madd(q1 * mq, q2, conj(s02));
+    E.e[2][1] = madd(q1 * mr, q2, conj(s12));
+    return E;
+}
+
+}  // namespace qcb

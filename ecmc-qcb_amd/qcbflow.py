@@ -1,0 +1,295 @@
+"""Python binding of libqcbflow (ctypes over the C ABI in include/qcbflow.h).
+
+The class `GradientFlow` mirrors the reference's C++ class of the same name
+(src/flow/gradient_flow.h:11-27) and the functions below mirror
+namespace mpi::observables (src/observables/observables_mpi.h:8-19) and
+read_ildg_clime / save_ildg_clime (src/io/ildg.h:17-20).  Every computation
+runs in the HIP kernels of libqcbflow.so; there is no CPU fallback: if the
+library is missing, `load()` raises.
+
+Link arrays are float64 numpy arrays with V*72 doubles in the reference's
+interior order (site x-fastest, mu, 3x3 row-major, re/im), e.g. of shape
+(V, 4, 3, 3, 2).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libqcbflow.so")
+
+QCBF_OK = 0
+QCBF_ERR_ARG = 1
+QCBF_ERR_HIP = 2
+QCBF_ERR_IO = 3
+QCBF_ERR_FORMAT = 4
+QCBF_ERR_OOM = 5
+QCBF_ERR_STATE = 6
+
+FP64 = 0
+FP32_MIXED = 1
+
+K_STAGE1, K_STAGE2, K_STAGE3, K_MEASURE = 0, 1, 2, 3
+K_COUNT = 4
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lp = ctypes.POINTER(ctypes.c_long)
+_vp = ctypes.c_void_p
+
+# name -> (restype, argtypes); every exported symbol of include/qcbflow.h
+SIGNATURES = {
+    "qcbf_last_error": (ctypes.c_char_p, [_vp]),
+    "qcbf_create": (ctypes.c_int, [_ip, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "qcbf_destroy": (ctypes.c_int, [_vp]),
+    "qcbf_get_dims": (ctypes.c_int, [_vp, _ip, _ip]),
+    "qcbf_set_links": (ctypes.c_int, [_vp, _dp]),
+    "qcbf_get_links": (ctypes.c_int, [_vp, _dp]),
+    "qcbf_hot_start": (ctypes.c_int, [_vp, ctypes.c_uint64]),
+    "qcbf_cold_start": (ctypes.c_int, [_vp]),
+    "qcbf_load_ildg": (ctypes.c_int, [_vp, ctypes.c_char_p]),
+    "qcbf_save_ildg": (ctypes.c_int, [_vp, ctypes.c_char_p]),
+    "qcbf_rk3_step": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int]),
+    "qcbf_compute_force": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "qcbf_compute_w1": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "qcbf_compute_w2": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "qcbf_replace_force_0": (ctypes.c_int, [_vp]),
+    "qcbf_compute_w3": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "qcbf_get_force": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "qcbf_plaquette": (ctypes.c_int, [_vp, _dp]),
+    "qcbf_q_e_clover": (ctypes.c_int, [_vp, _dp, _dp]),
+    "qcbf_measure": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
+    "qcbf_flow_measure": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp,
+                                         _dp]),
+    "qcbf_exp_warnings": (ctypes.c_int, [_vp, _ip]),
+    "qcbf_unitarity": (ctypes.c_int, [_vp, _dp, _dp]),
+    "qcbf_synchronize": (ctypes.c_int, [_vp]),
+    "qcbf_su3_exp": (ctypes.c_int, [_dp, ctypes.c_int, ctypes.c_double, _dp, _ip]),
+    "qcbf_timing_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "qcbf_timing_read": (ctypes.c_int, [_vp, _dp, _lp]),
+    "qcbf_copy_bench": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_int, _dp]),
+    "qcbf_device_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, _ip]),
+}
+
+_lib = None
+
+
+class QcbfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"qcbflow error {code}: {msg}")
+        self.code = code
+
+
+def load():
+    """Load libqcbflow.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it first (__graft_entry__.build())")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def _check(rc, handle=None):
+    if rc != QCBF_OK:
+        msg = load().qcbf_last_error(handle)
+        raise QcbfError(rc, msg.decode() if msg else "")
+
+
+def _p(a):
+    if a.dtype != np.float64 or not a.flags.c_contiguous:
+        raise ValueError("expected a C-contiguous float64 array")
+    return a.ctypes.data_as(_dp)
+
+
+def _dims(L):
+    if np.isscalar(L):
+        L = (L, L, L, L)
+    if len(L) != 4:
+        raise ValueError("lattice extents must be 4 integers (Lx, Ly, Lz, Lt)")
+    return tuple(int(v) for v in L)
+
+
+def device_info():
+    name = ctypes.create_string_buffer(256)
+    cus = ctypes.c_int()
+    _check(load().qcbf_device_info(name, 256, ctypes.byref(cus)))
+    return name.value.decode(), cus.value
+
+
+def su3_exp(Z, coeff=1.0):
+    """exp_su3_luscher(Z, coeff) on the device for (n, 3, 3) complex Z.
+    Returns (exp, flagged)."""
+    Z = np.asarray(Z, dtype=np.complex128)
+    shape = Z.shape
+    Zr = np.ascontiguousarray(np.stack([Z.real, Z.imag], axis=-1).reshape(-1, 18))
+    out = np.empty_like(Zr)
+    flagged = ctypes.c_int()
+    _check(load().qcbf_su3_exp(_p(Zr), Zr.shape[0], float(coeff), _p(out), ctypes.byref(flagged)))
+    o = out.reshape(-1, 3, 3, 2)
+    return (o[..., 0] + 1j * o[..., 1]).reshape(shape), bool(flagged.value)
+
+
+class GradientFlow:
+    """Device-resident Wilson flow of one SU(3) lattice (the reference's
+    GradientFlow, src/flow/gradient_flow.h:11-27, holding its field_c on the
+    GPU).  `epsilon` is the flow step used by rk3_step / topo_charge_flowed."""
+
+    def __init__(self, L, epsilon=0.02, precision=FP64):
+        self.L = _dims(L)
+        self.V = int(np.prod(self.L))
+        self.epsilon = float(epsilon)
+        self.precision = precision
+        h = _vp()
+        _check(load().qcbf_create((ctypes.c_int * 4)(*self.L), int(precision), ctypes.byref(h)))
+        self._h = h
+
+    # -- lifetime
+    def close(self):
+        if getattr(self, "_h", None):
+            load().qcbf_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _c(self, rc):
+        _check(rc, self._h)
+
+    # -- field in / out
+    def copy(self, links):
+        """GradientFlow::copy: upload V*72 doubles (reference order)."""
+        links = np.ascontiguousarray(links, dtype=np.float64)
+        if links.size != self.V * 72:
+            raise ValueError(f"expected {self.V * 72} doubles, got {links.size}")
+        self._c(load().qcbf_set_links(self._h, _p(links)))
+
+    set_links = copy
+
+    def field(self):
+        """field_c read back, shape (V, 4, 3, 3, 2)."""
+        out = np.empty((self.V, 4, 3, 3, 2))
+        self._c(load().qcbf_get_links(self._h, _p(out)))
+        return out
+
+    def hot_start(self, seed):
+        self._c(load().qcbf_hot_start(self._h, int(seed)))
+
+    def cold_start(self):
+        self._c(load().qcbf_cold_start(self._h))
+
+    def load_ildg(self, path):
+        self._c(load().qcbf_load_ildg(self._h, os.fsencode(path)))
+
+    def save_ildg(self, path):
+        self._c(load().qcbf_save_ildg(self._h, os.fsencode(path)))
+
+    # -- flow
+    def rk3_step(self, n=1, eps=None):
+        self._c(load().qcbf_rk3_step(self._h, self.epsilon if eps is None else float(eps), int(n)))
+
+    def compute_force(self, i):
+        self._c(load().qcbf_compute_force(self._h, int(i)))
+
+    def compute_w1(self):
+        self._c(load().qcbf_compute_w1(self._h, self.epsilon))
+
+    def compute_w2(self):
+        self._c(load().qcbf_compute_w2(self._h, self.epsilon))
+
+    def replace_force_0(self):
+        self._c(load().qcbf_replace_force_0(self._h))
+
+    def compute_w3(self):
+        self._c(load().qcbf_compute_w3(self._h, self.epsilon))
+
+    def force(self, i):
+        out = np.empty((self.V, 4, 3, 3, 2))
+        self._c(load().qcbf_get_force(self._h, int(i), _p(out)))
+        return out
+
+    # -- observables
+    def plaquette(self):
+        p = ctypes.c_double()
+        self._c(load().qcbf_plaquette(self._h, ctypes.byref(p)))
+        return p.value
+
+    def q_e_clover(self):
+        q, e = ctypes.c_double(), ctypes.c_double()
+        self._c(load().qcbf_q_e_clover(self._h, ctypes.byref(q), ctypes.byref(e)))
+        return q.value, e.value
+
+    def measure(self):
+        """(P, E, Q) in one fused pass."""
+        p, e, q = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._c(load().qcbf_measure(self._h, ctypes.byref(p), ctypes.byref(e), ctypes.byref(q)))
+        return p.value, e.value, q.value
+
+    def flow_measure(self, n_steps_gf, n_rk_steps, eps=None):
+        """topo_charge_flowed cadence: returns (tQE of shape (n, 3), plaquette (n,))."""
+        tqe = np.empty(3 * n_steps_gf)
+        plaq = np.empty(n_steps_gf)
+        self._c(load().qcbf_flow_measure(self._h, self.epsilon if eps is None else float(eps),
+                                         int(n_steps_gf), int(n_rk_steps), _p(tqe), _p(plaq)))
+        return tqe.reshape(-1, 3), plaq
+
+    def exp_warnings(self):
+        f = ctypes.c_int()
+        self._c(load().qcbf_exp_warnings(self._h, ctypes.byref(f)))
+        return f.value
+
+    def unitarity(self):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._c(load().qcbf_unitarity(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def synchronize(self):
+        self._c(load().qcbf_synchronize(self._h))
+
+    # -- measurement support
+    def timing(self, on=True):
+        self._c(load().qcbf_timing_enable(self._h, 1 if on else 0))
+
+    def timing_read(self):
+        ms = (ctypes.c_double * K_COUNT)()
+        n = (ctypes.c_long * K_COUNT)()
+        self._c(load().qcbf_timing_read(self._h, ms, n))
+        return list(ms), list(n)
+
+    def copy_bench(self, nbytes, reps=10):
+        ms = ctypes.c_double()
+        self._c(load().qcbf_copy_bench(self._h, int(nbytes), int(reps), ctypes.byref(ms)))
+        return ms.value
+
+
+# ---- mpi::observables mirrors (single device; the topology argument of the
+# reference is dropped because one GPU holds the whole lattice) -------------
+def mean_plaquette_global(flow):
+    return flow.plaquette()
+
+
+def topo_q_e_clover_global(flow):
+    return flow.q_e_clover()
+
+
+def topo_charge_flowed(links, flow, N_steps_gf, N_rk_steps):
+    """Copies `links` into the flow (gf.copy) and returns the flat tQE vector
+    of 3*N_steps_gf doubles, as observables_mpi.cpp:159-186."""
+    flow.copy(links)
+    tqe, _ = flow.flow_measure(N_steps_gf, N_rk_steps)
+    return tqe.reshape(-1)

@@ -1,0 +1,172 @@
+/*
+ * qcbflow.h -- C ABI of libqcbflow, the MI355X (gfx950) Wilson-flow
+ * measurement library.
+ *
+ * Replaces the Wilson-flow measurement path of ECMC-QCB:
+ *   class GradientFlow            src/flow/gradient_flow.h:11-27
+ *   namespace mpi::observables    src/observables/observables_mpi.h:8-19
+ *   read_ildg_clime / save_ildg_clime  src/io/ildg.h:17-20
+ *   exp_su3_luscher               src/su3/utils.h:45
+ * The C++ drop-in headers (ecmc-qcb_amd/compat/) keep those exact signatures
+ * on top of this ABI.
+ *
+ * Conventions
+ *  - Every function returns an int status (QCBF_OK = 0).  On failure
+ *    qcbf_last_error(h) (or qcbf_last_error(NULL) for calls without a handle)
+ *    returns a message.  Nothing throws across this boundary.
+ *  - Host arrays of links use the reference's interior order, which is also
+ *    the ILDG payload order: site s = ((t*Lz + z)*Ly + y)*Lx + x (x fastest),
+ *    then mu = 0..3, then the 3x3 matrix row-major, each entry (re, im):
+ *    72 doubles per site (src/io/ildg.cpp:233-246).
+ *  - A handle owns all of its device memory and one HIP stream; calls on one
+ *    handle are ordered on that stream.  A handle is not thread-safe: use one
+ *    per host thread.  Host arrays are only read/written during the call.
+ *  - Observables use the reference's conventions (observables_mpi.cpp):
+ *      P = <Re tr U_p>/3 over the 6 planes and all sites,
+ *      E = (1/V) sum_x 1/2 sum_{mu<nu} |F_munu(x)|_F^2,
+ *      Q = (1/4pi^2) sum_x [Re tr F01F23 - Re tr F02F13 + Re tr F03F12],
+ *    with F = (C - C^dag)/8 from the four-leaf clover C.
+ */
+#ifndef QCBFLOW_H
+#define QCBFLOW_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QCBFLOW_ABI_VERSION 1
+
+enum qcbf_status {
+    QCBF_OK = 0,
+    QCBF_ERR_ARG = 1,     /* invalid argument (null pointer, bad extent, ...) */
+    QCBF_ERR_HIP = 2,     /* HIP runtime error */
+    QCBF_ERR_IO = 3,      /* file cannot be opened / read / written */
+    QCBF_ERR_FORMAT = 4,  /* malformed LIME/ILDG file or size mismatch */
+    QCBF_ERR_OOM = 5,     /* device or pinned host allocation failed */
+    QCBF_ERR_STATE = 6    /* call not valid in the handle's current state */
+};
+
+enum qcbf_precision {
+    QCBF_FP64 = 0,       /* links and arithmetic in FP64 (plain path) */
+    QCBF_FP32_MIXED = 1  /* two-row FP32 links, FP32 flow arithmetic, FP64
+                            Cayley-Hamilton coefficients, projection and sums */
+};
+
+/* Kernel classes reported by qcbf_timing_read. */
+enum qcbf_kernel_class {
+    QCBF_K_STAGE1 = 0,  /* fused RK3 stage 1 */
+    QCBF_K_STAGE2 = 1,  /* fused RK3 stage 2 */
+    QCBF_K_STAGE3 = 2,  /* fused RK3 stage 3 */
+    QCBF_K_MEASURE = 3, /* clover/plaquette/Q/E measurement + its reduction */
+    QCBF_K_COUNT = 4
+};
+
+typedef struct qcbf_flow qcbf_flow;
+
+/* Message of the last failed call on h (h == NULL: last failure on this thread
+ * of a call made without a handle, or of qcbf_create). */
+const char* qcbf_last_error(const qcbf_flow* h);
+
+/* GradientFlow(double epsilon, const GaugeField&, const GeometryCB&)
+ * (src/flow/gradient_flow.cpp:12-13) + GeometryCB(L) (src/geometry/GeometryCB.cpp:5-9).
+ * dims = {Lx, Ly, Lz, Lt}, each >= 2, product < 2^28 sites.  Allocates the link
+ * field (cold start), its RK3 companion buffers and the measurement buffers on
+ * the current HIP device. */
+int qcbf_create(const int dims[4], int precision, qcbf_flow** out);
+int qcbf_destroy(qcbf_flow* h);
+int qcbf_get_dims(const qcbf_flow* h, int dims[4], int* precision);
+
+/* GradientFlow::copy(const GaugeField&) (src/flow/gradient_flow.cpp:138):
+ * upload V*72 doubles in the reference order into the flowing field. */
+int qcbf_set_links(qcbf_flow* h, const double* links);
+/* Read the flowing field back (GradientFlow::field_c). */
+int qcbf_get_links(qcbf_flow* h, double* links);
+
+/* GaugeField::hot_start / cold_start (src/gauge/GaugeField.cpp:10-33):
+ * Haar-random SU(3) links from a counter-based generator keyed by seed, or the
+ * identity.  Generated on the device. */
+int qcbf_hot_start(qcbf_flow* h, uint64_t seed);
+int qcbf_cold_start(qcbf_flow* h);
+
+/* read_ildg_clime (src/io/ildg.cpp:138-237): load an ILDG/LIME file straight
+ * into the flowing field.  path is the file itself (the reference's
+ * dirpath/filename/filename composition lives in the C++ drop-in).  The
+ * ildg-format record, when present, must match the handle's extents;
+ * precision 64 and 32 payloads are accepted; the ildg-binary-data size must
+ * be V*72*precision/8 bytes.  Errors: QCBF_ERR_IO, QCBF_ERR_FORMAT. */
+int qcbf_load_ildg(qcbf_flow* h, const char* path);
+/* save_ildg_clime (src/io/ildg.cpp:34-136): write the flowing field as a
+ * 64-bit big-endian ILDG file with the reference's record framing
+ * (ildg-format MB=1 ME=1, then ildg-binary-data MB=0 ME=1). */
+int qcbf_save_ildg(qcbf_flow* h, const char* path);
+
+/* GradientFlow::rk3_step (src/flow/gradient_flow.cpp:124-135), n_steps times,
+ * with flow step eps.  Three fused kernels per step; asynchronous on the
+ * handle's stream. */
+int qcbf_rk3_step(qcbf_flow* h, double eps, int n_steps);
+
+/* The unfused GradientFlow member functions, for callers that drive the RK3
+ * stages themselves (src/flow/gradient_flow.cpp:17-121).  Forces are kept as
+ * su(3) fields of 8 reals per link. */
+int qcbf_compute_force(qcbf_flow* h, int i);   /* i = 0 -> force_0, 1 -> force_1 */
+int qcbf_compute_w1(qcbf_flow* h, double eps); /* U <- exp(eps/4 F0) U */
+int qcbf_compute_w2(qcbf_flow* h, double eps); /* U <- exp(8/9 eps F1 - 17/36 eps F0) U */
+int qcbf_replace_force_0(qcbf_flow* h);        /* F0 <- 8/9 F1 - 17/36 F0 */
+int qcbf_compute_w3(qcbf_flow* h, double eps); /* U <- exp(3/4 eps F1 - eps F0) U */
+/* Read force_i back as full 3x3 matrices in the link order (V*72 doubles). */
+int qcbf_get_force(qcbf_flow* h, int i, double* out);
+
+/* mean_plaquette_global (src/observables/observables_mpi.cpp:39-48). */
+int qcbf_plaquette(qcbf_flow* h, double* plaq);
+/* topo_q_e_clover_global (src/observables/observables_mpi.cpp:136-156):
+ * Q and E of the flowing field. */
+int qcbf_q_e_clover(qcbf_flow* h, double* Q, double* E);
+/* Both in one fused pass. */
+int qcbf_measure(qcbf_flow* h, double* plaq, double* E, double* Q);
+
+/* topo_charge_flowed (src/observables/observables_mpi.cpp:159-186) on the
+ * current flowing field: for n = 0..n_steps_gf-1 measure (t, Q, E) at t
+ * (tQE[3n..3n+2]) and the plaquette (plaq[n], may be NULL); unless n is the
+ * last, run n_rk_steps RK3 steps and t += eps after each.  Results stay on the
+ * device until the end of the call (one synchronisation).  The flowing field
+ * holds the flowed links afterwards. */
+int qcbf_flow_measure(qcbf_flow* h, double eps, int n_steps_gf, int n_rk_steps, double* tQE,
+                      double* plaq);
+
+/* Number of exponentials since the last call with t = -1/2 tr X^2 > 1 flagged
+ * (exp_su3_luscher warns on stderr, src/su3/utils.cpp:153-155): 0 or 1
+ * per kernel launch; reading resets the count. */
+int qcbf_exp_warnings(qcbf_flow* h, int* flagged);
+
+/* max_{links,ij} |(U U^dag - 1)_ij| and max |det U - 1| of the flowing field
+ * (a NaN anywhere reports +inf). */
+int qcbf_unitarity(qcbf_flow* h, double* max_dev, double* max_det_dev);
+
+/* Wait for all work queued on the handle's stream. */
+int qcbf_synchronize(qcbf_flow* h);
+
+/* exp_su3_luscher(Z, coeff) (src/su3/utils.cpp:145-193) on the device for n
+ * traceless anti-hermitian 3x3 matrices (18 doubles each, row-major re/im).
+ * *flagged (may be NULL) is set to 1 if any had t > 1. */
+int qcbf_su3_exp(const double* Z, int n, double coeff, double* out, int* flagged);
+
+/* ---- measurement support (no counterpart in the reference) -------------- */
+/* When on, every fused-stage and measurement launch on h is bracketed by HIP
+ * events on h's stream; qcbf_timing_read synchronises and returns, per kernel
+ * class, the summed kernel milliseconds and launch count since timing was
+ * (re)enabled. */
+int qcbf_timing_enable(qcbf_flow* h, int on);
+int qcbf_timing_read(qcbf_flow* h, double total_ms[QCBF_K_COUNT], long launches[QCBF_K_COUNT]);
+/* Device-to-device copy of `bytes` through a plain copy kernel, repeated reps
+ * times; *ms = average kernel milliseconds (HIP events). */
+int qcbf_copy_bench(qcbf_flow* h, size_t bytes, int reps, double* ms);
+/* Name and compute-unit count of the current device. */
+int qcbf_device_info(char* name, int name_len, int* compute_units);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QCBFLOW_H */

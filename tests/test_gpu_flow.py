@@ -1,0 +1,216 @@
+"""GPU parity of the FP64 flow path against the oracle, through the C ABI.
+
+Tolerances (FP64 GPU vs FP64 CPU oracle): plaquette and E within 1e-12
+relative, Q within 1e-10 absolute, at every measured flow time."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import fields
+import oracle
+import qcbflow
+
+pytestmark = pytest.mark.gpu
+
+TOL_REL_P = 1e-12
+TOL_REL_E = 1e-12
+TOL_ABS_Q = 1e-10
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def thermal(L, seed, sweeps, beta=6.0):
+    f = oracle.hot_start(L, seed)
+    return oracle.heatbath(f, L, beta, sweeps, seed + 0x5EED)
+
+
+def check_traj(tqe, plaq, ref_tqe, ref_plaq):
+    assert np.array_equal(tqe[:, 0], ref_tqe[:, 0])
+    dP = np.max(np.abs(plaq - ref_plaq) / np.abs(ref_plaq))
+    dE = np.max(np.abs(tqe[:, 2] - ref_tqe[:, 2]) / np.abs(ref_tqe[:, 2]))
+    dQ = np.max(np.abs(tqe[:, 1] - ref_tqe[:, 1]))
+    say(f"  max rel dP={dP:.2e} rel dE={dE:.2e} abs dQ={dQ:.2e}")
+    assert dP < TOL_REL_P and dE < TOL_REL_E and dQ < TOL_ABS_Q
+
+
+def test_device_present():
+    name, cus = qcbflow.device_info()
+    say(f"device: {name}, {cus} CUs")
+    assert cus > 0
+
+
+def test_exp_matches_scipy_golden_and_oracle(golden_dir):
+    say("exp: 64 golden cases")
+    with open(os.path.join(golden_dir, "expm_golden.json")) as f:
+        cases = json.load(f)["cases"]
+    for c in cases:
+        Z = np.array(c["Z"]).reshape(3, 3, 2)
+        Z = Z[..., 0] + 1j * Z[..., 1]
+        E = np.array(c["expm"]).reshape(3, 3, 2)
+        E = E[..., 0] + 1j * E[..., 1]
+        got, flagged = qcbflow.su3_exp(Z[None], c["coeff"])
+        assert not flagged
+        assert np.abs(got[0] - E).max() < 2e-14
+        assert np.abs(got[0] - oracle.exp_su3(Z[None], c["coeff"])[0]).max() < 1e-15
+
+
+def test_exp_flags_large_argument():
+    rng = np.random.default_rng(1)
+    X = fields.random_algebra(rng, 4)
+    X *= (2.0 / np.linalg.norm(X, axis=(1, 2)))[:, None, None]  # t = 2
+    _, flagged = qcbf_flag(X)
+    assert flagged
+
+
+def qcbf_flag(X):
+    return qcbflow.su3_exp(X, 1.0)
+
+
+@pytest.mark.parametrize("L", [(4, 4, 4, 4), (3, 5, 4, 6)])
+def test_force_parity(L):
+    say(f"force parity {L}")
+    f = oracle.hot_start(L, 17)
+    ref = oracle.compute_force(f, L)
+    with qcbflow.GradientFlow(L) as gf:
+        gf.copy(f)
+        gf.compute_force(0)
+        got = gf.force(0)
+    assert np.abs(got - ref).max() < 1e-13
+
+
+@pytest.mark.parametrize("L,n", [((4, 6, 4, 8), [1, 2, -1, 1, 3, 2]), ((5, 4, 6, 4), [1, 0, 2, 0, 1, 1])])
+def test_measure_closed_forms(L, n):
+    say(f"abelian closed form {L}")
+    f, phis = fields.abelian_flux(L, n)
+    P, E, Q = fields.abelian_expected(L, phis)
+    with qcbflow.GradientFlow(L) as gf:
+        gf.copy(f)
+        p, e, q = gf.measure()
+        assert abs(gf.plaquette() - P) < 1e-14
+    assert abs(p - P) < 1e-14 and abs(e - E) < 1e-13 * max(1, E) and abs(q - Q) < 1e-11 * max(1, abs(Q))
+
+
+def test_unfused_stages_equal_fused_step():
+    """compute_force/w1/w2/replace_force_0/w3 in the reference's order give the
+    same field as the fused rk3_step."""
+    L = (4, 4, 6, 4)
+    say(f"unfused vs fused {L}")
+    f = thermal(L, 3, 3)
+    with qcbflow.GradientFlow(L, 0.02) as a, qcbflow.GradientFlow(L, 0.02) as b:
+        a.copy(f)
+        b.copy(f)
+        for _ in range(2):
+            a.rk3_step()
+            b.compute_force(0)
+            b.compute_w1()
+            b.compute_force(1)
+            b.compute_w2()
+            b.replace_force_0()
+            b.compute_force(1)
+            b.compute_w3()
+        fa, fb = a.field(), b.field()
+    ref = f.copy()
+    oracle.rk3(ref, L, 0.02, 2)
+    assert np.abs(fa - fb).max() < 1e-14
+    assert np.abs(fa - ref).max() < 1e-13
+
+
+def test_flow_parity_8x8x8x8_thermalised():
+    """Config: 8^4, Haar hot start seed 1234, 20 heatbath sweeps at beta = 6.0;
+    N_steps_gf = 100, N_rk_steps = 1, eps = 0.02 (99 RK3 steps, last t = 1.98)."""
+    L = (8, 8, 8, 8)
+    say("flow parity 8^4 x 99 RK3 steps")
+    f = thermal(L, 1234, 20)
+    ref_tqe, ref_plaq, ref_f = oracle.flow_measure(f, L, 0.02, 100, 1, want_field=True)
+    with qcbflow.GradientFlow(L, 0.02) as gf:
+        gf.copy(f)
+        tqe, plaq = gf.flow_measure(100, 1)
+        out = gf.field()
+        assert gf.exp_warnings() == 0
+    check_traj(tqe, plaq, ref_tqe, ref_plaq)
+    say(f"  max |dU| = {np.abs(out - ref_f).max():.2e}")
+    assert np.abs(out - ref_f).max() < 1e-12
+
+
+@pytest.mark.parametrize("L", [(6, 4, 5, 8), (3, 5, 4, 7), (12, 4, 4, 4)])
+def test_flow_parity_anisotropic(L):
+    say(f"flow parity {L}")
+    f = thermal(L, 77, 5)
+    ref_tqe, ref_plaq, _ = oracle.flow_measure(f, L, 0.02, 11, 2)
+    with qcbflow.GradientFlow(L, 0.02) as gf:
+        gf.copy(f)
+        tqe, plaq = gf.flow_measure(11, 2)
+    check_traj(tqe, plaq, ref_tqe, ref_plaq)
+
+
+def test_flow_is_deterministic():
+    L = (8, 8, 8, 8)
+    say("determinism 8^4")
+    with qcbflow.GradientFlow(L, 0.02) as gf:
+        gf.hot_start(5)
+        f = gf.field()
+        a, pa = gf.flow_measure(6, 1)
+        gf.copy(f)
+        b, pb = gf.flow_measure(6, 1)
+    assert np.array_equal(a, b) and np.array_equal(pa, pb)
+
+
+def test_hot_start_is_su3_and_haar_like():
+    L = (8, 8, 8, 8)
+    with qcbflow.GradientFlow(L) as gf:
+        gf.hot_start(42)
+        du, dd = gf.unitarity()
+        f = gf.field()
+        p = gf.plaquette()
+    assert du < 1e-14 and dd < 1e-14
+    # Haar: <Re tr U> = 0, <|tr U|^2> = 1; plaquette of random links ~ 0
+    tr = np.trace(fields.to_cplx(f).reshape(-1, 3, 3), axis1=1, axis2=2)
+    assert abs(np.mean(tr.real)) < 0.02 and abs(np.mean(np.abs(tr) ** 2) - 1) < 0.03
+    assert abs(p) < 0.01
+
+
+def test_exp_warning_flag_on_flow():
+    L = (4, 4, 4, 4)
+    with qcbflow.GradientFlow(L, 0.5) as gf:
+        gf.hot_start(1)
+        gf.rk3_step(1)
+        assert gf.exp_warnings() == 1
+        assert gf.exp_warnings() == 0  # reading resets
+
+
+def test_large_lattice_properties_32x32x32x32():
+    """Size-independent properties at the full 32^4 workload: one RK3 step
+    against the oracle, unitarity after 10 steps, translation and gauge
+    invariance of P, E, Q."""
+    L = (32, 32, 32, 32)
+    say("32^4: hot start, 1 RK3 step vs oracle")
+    with qcbflow.GradientFlow(L, 0.02) as gf:
+        gf.hot_start(7)
+        f = gf.field()
+        gf.rk3_step(1)
+        g1 = gf.field()
+        p1, e1, q1 = gf.measure()
+        ref = f.copy()
+        oracle.rk3(ref, L, 0.02, 1)
+        assert np.abs(g1 - ref).max() < 1e-13
+        P, (Q, E) = oracle.plaquette(ref, L), oracle.q_e_clover(ref, L)
+        assert abs(p1 - P) < 1e-12 * abs(P) and abs(e1 - E) < 1e-12 * E and abs(q1 - Q) < 1e-10
+        say("32^4: 9 more steps, unitarity")
+        gf.rk3_step(9)
+        du, dd = gf.unitarity()
+        assert du < 1e-13 and dd < 1e-13
+        p0, e0, q0 = gf.measure()
+        g = gf.field()
+        say("32^4: translation")
+        gf.copy(fields.translate(g, L, (3, 1, 4, 1)))
+        p2, e2, q2 = gf.measure()
+        assert abs(p2 - p0) < 1e-13 and abs(e2 - e0) < 1e-12 * e0 and abs(q2 - q0) < 1e-10
+        say("32^4: gauge transformation")
+        G = fields.random_su3(np.random.default_rng(3), int(np.prod(L)))
+        gf.copy(fields.gauge_apply(g, L, G))
+        p3, e3, q3 = gf.measure()
+        assert abs(p3 - p0) < 1e-13 and abs(e3 - e0) < 1e-12 * e0 and abs(q3 - q0) < 1e-10
