@@ -143,11 +143,14 @@ __global__ __launch_bounds__(256) void k_hot_start64(double2* __restrict__ U, un
         z[k / 3][k % 3] = {rad * cs, rad * sn};
     }
     for (int j = 0; j < 3; j++) {
-        for (int p = 0; p < j; p++) {
-            cpx<double> dot = {0.0, 0.0};
-            for (int i = 0; i < 3; i++) dot = madd_cl(dot, z[i][p], z[i][j]);
-            for (int i = 0; i < 3; i++) z[i][j] = z[i][j] - dot * z[i][p];
-        }
+        // classical Gram-Schmidt applied twice ("twice is enough") keeps the
+        // columns orthogonal to rounding even for ill-conditioned draws
+        for (int pass = 0; pass < 2; pass++)
+            for (int p = 0; p < j; p++) {
+                cpx<double> dot = {0.0, 0.0};
+                for (int i = 0; i < 3; i++) dot = madd_cl(dot, z[i][p], z[i][j]);
+                for (int i = 0; i < 3; i++) z[i][j] = z[i][j] - dot * z[i][p];
+            }
         double nn = 0.0;
         for (int i = 0; i < 3; i++) nn += norm2(z[i][j]);
         const double inv = 1.0 / sqrt(nn);
