@@ -22,6 +22,8 @@ namespace qcb {
 // Staple sum of U_mu(x):
 //   sum_{nu != mu} U_nu(x+mu) (U_nu(x) U_mu(x+nu))^dag
 //                + (U_mu(x-nu) U_nu(x+mu-nu))^dag U_nu(x-nu)
+// One operand of every product is streamed from memory a row at a time, so
+// only the accumulator, one product and one full operand are live at once.
 template <int MU>
 __device__ inline mat3<double> staple_sum64(const field64& f, unsigned s, const nbr& n) {
     mat3<double> st = zero3<double>();
@@ -32,13 +34,47 @@ __device__ inline mat3<double> staple_sum64(const field64& f, unsigned s, const 
         const unsigned xnu = s + n.up[nu];
         const unsigned xmnu = s + n.dn[nu];
         const unsigned xmunu = xmu + n.dn[nu];
-        {
-            const mat3<double> a = mul(f.link(nu, s), f.link(MU, xnu));
-            acc_mul_nd(st, f.link(nu, xmu), a);
+        {  // T = U_nu(x) U_mu(x+nu); st += U_nu(x+mu) T^dag
+            const mat3<double> B = f.link(MU, xnu);
+            mat3<double> T;
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                cpx<double> a[3];
+                f.row(nu, s, i, a);
+#pragma unroll
+                for (int j = 0; j < 3; j++)
+                    T.e[i][j] = madd(madd(a[0] * B.e[0][j], a[1], B.e[1][j]), a[2], B.e[2][j]);
+            }
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                cpx<double> c[3];
+                f.row(nu, xmu, i, c);
+#pragma unroll
+                for (int j = 0; j < 3; j++)
+                    st.e[i][j] = madd_c(madd_c(madd_c(st.e[i][j], c[0], T.e[j][0]), c[1], T.e[j][1]),
+                                        c[2], T.e[j][2]);
+            }
         }
-        {
-            const mat3<double> b = mul(f.link(MU, xmnu), f.link(nu, xmunu));
-            acc_mul_dn(st, b, f.link(nu, xmnu));
+        {  // T = U_mu(x-nu) U_nu(x+mu-nu); st += T^dag U_nu(x-nu)
+            const mat3<double> B = f.link(nu, xmunu);
+            mat3<double> T;
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                cpx<double> a[3];
+                f.row(MU, xmnu, i, a);
+#pragma unroll
+                for (int j = 0; j < 3; j++)
+                    T.e[i][j] = madd(madd(a[0] * B.e[0][j], a[1], B.e[1][j]), a[2], B.e[2][j]);
+            }
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                cpx<double> v[3];
+                f.row(nu, xmnu, k, v);  // row k of U_nu(x-nu)
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++) st.e[i][j] = madd_cl(st.e[i][j], T.e[k][i], v[j]);
+            }
         }
     }
     return st;
@@ -77,13 +113,18 @@ __device__ inline void stage_body64(const field64& f, double2* __restrict__ Uout
     store_link64(Uout, f.V, MU, s, mul(e, u));
 }
 
+#ifndef QCB_STAGE_WAVES_PER_EU
+#define QCB_STAGE_WAVES_PER_EU 1
+#endif
+
 template <int STAGE>
-__global__ __launch_bounds__(256) void k_rk3_stage64(const double2* __restrict__ Uin,
-                                                     double2* __restrict__ Uout,
-                                                     double2* __restrict__ acc, lattice g,
-                                                     double cz, double ca, int* __restrict__ warn) {
-    const unsigned s = blockIdx.x * kSitesPerBlock + threadIdx.x;
-    if (s >= g.V) return;
+__global__ __launch_bounds__(256, QCB_STAGE_WAVES_PER_EU) void k_rk3_stage64(
+    const double2* __restrict__ Uin, double2* __restrict__ Uout, double2* __restrict__ acc,
+    lattice g, double cz, double ca, int* __restrict__ warn) {
+    const unsigned ntiles = (g.V + kSitesPerBlock - 1) / kSitesPerBlock;
+    const unsigned tile = tile_of_block(blockIdx.x, ntiles, g.map_mode, g.map_win);
+    const unsigned s = tile * kSitesPerBlock + threadIdx.x;
+    if (tile >= ntiles || s >= g.V) return;
     const nbr n = neighbours(g, s);
     const field64 f{Uin, g.V};
     bool w = false;
@@ -100,7 +141,8 @@ hipError_t launch_rk3_stage_f64(int stage, const double2* Uin, double2* Uout, do
                                 const lattice& g, double cz, double ca, int* warn,
                                 hipStream_t st) {
     const dim3 block(kSitesPerBlock, 4);
-    const dim3 grid((g.V + kSitesPerBlock - 1) / kSitesPerBlock);
+    const dim3 grid(grid_for_tiles((g.V + kSitesPerBlock - 1) / kSitesPerBlock, g.map_mode,
+                                   g.map_win));
     switch (stage) {
         case 1: k_rk3_stage64<1><<<grid, block, 0, st>>>(Uin, Uout, acc, g, cz, ca, warn); break;
         case 2: k_rk3_stage64<2><<<grid, block, 0, st>>>(Uin, Uout, acc, g, cz, ca, warn); break;

@@ -24,7 +24,36 @@ struct lattice {
     int L[4];           // Lx, Ly, Lz, Lt
     unsigned V;         // number of sites
     unsigned stride[4]; // 1, Lx, Lx*Ly, Lx*Ly*Lz
+    // workgroup -> site-tile order of the flow kernels (see tile_of_block)
+    int map_mode;       // 0 linear, 1 one contiguous chunk per XCD, 2 XCD chunks per window
+    unsigned map_win;   // window in workgroups for map_mode 2 (multiple of 8)
 };
+
+// The dispatcher hands workgroups to the 8 XCDs round-robin (workgroup b runs
+// on XCD b % 8), and each XCD has its own L2.  These orders give every XCD a
+// contiguous run of site tiles so that neighbouring tiles share its L2:
+//   mode 1: XCD k takes tiles [k q, (k+1) q), q = ceil(ntiles / 8);
+//   mode 2: the same inside consecutive windows of map_win workgroups, so all
+//           XCDs still sweep the lattice together (t-neighbours stay in the
+//           shared last-level cache).
+// Returns a tile index >= ntiles for padding workgroups, which exit.
+__host__ __device__ inline unsigned tile_of_block(unsigned b, unsigned ntiles, int mode,
+                                                  unsigned win) {
+    if (mode == 1) {
+        const unsigned q = (ntiles + 7) / 8;
+        return (b % 8) * q + b / 8;
+    }
+    if (mode == 2) {
+        const unsigned w = b / win, j = b % win;
+        return w * win + (j % 8) * (win / 8) + j / 8;
+    }
+    return b;
+}
+inline unsigned grid_for_tiles(unsigned ntiles, int mode, unsigned win) {
+    if (mode == 1) return 8 * ((ntiles + 7) / 8);
+    if (mode == 2) return ((ntiles + win - 1) / win) * win;
+    return ntiles;
+}
 
 inline lattice make_lattice(const int dims[4]) {
     lattice g;
@@ -35,6 +64,8 @@ inline lattice make_lattice(const int dims[4]) {
         st *= (unsigned)dims[d];
     }
     g.V = st;
+    g.map_mode = 0;
+    g.map_win = 64;
     return g;
 }
 
@@ -77,6 +108,15 @@ struct field64 {
             m.e[k / 3][k % 3] = {v.x, v.y};
         }
         return m;
+    }
+    // row i of U_mu(s)
+    __device__ inline void row(int mu, unsigned s, int i, cpx<double> (&r)[3]) const {
+        const double2* b = p + ((size_t)mu * 9 + 3 * i) * V + s;
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            double2 v = b[(size_t)j * V];
+            r[j] = {v.x, v.y};
+        }
     }
 };
 

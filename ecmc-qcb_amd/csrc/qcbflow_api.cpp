@@ -577,6 +577,33 @@ extern "C" int qcbf_su3_exp(const double* Z, int n, double coeff, double* out, i
     return rc;
 }
 
+// ---- launch tuning -------------------------------------------------------------------
+extern "C" int qcbf_tune(qcbf_flow* h, int key, long value) {
+    CHECK_H(h);
+    switch (key) {
+        case QCBF_TUNE_MAP_MODE:
+            if (value < 0 || value > 2) return fail(h, QCBF_ERR_ARG, "map mode %ld not in 0..2", value);
+            h->g.map_mode = (int)value;
+            return QCBF_OK;
+        case QCBF_TUNE_MAP_WIN:
+            if (value < 8 || value % 8 != 0 || value > (1L << 24))
+                return fail(h, QCBF_ERR_ARG, "map window %ld not a multiple of 8 in [8, 2^24]", value);
+            h->g.map_win = (unsigned)value;
+            return QCBF_OK;
+        default:
+            return fail(h, QCBF_ERR_ARG, "unknown tuning key %d", key);
+    }
+}
+
+extern "C" int qcbf_get_tune(const qcbf_flow* h, int key, long* value) {
+    if (!h || !value) return fail(nullptr, QCBF_ERR_ARG, "null argument");
+    switch (key) {
+        case QCBF_TUNE_MAP_MODE: *value = h->g.map_mode; return QCBF_OK;
+        case QCBF_TUNE_MAP_WIN: *value = h->g.map_win; return QCBF_OK;
+        default: return fail(nullptr, QCBF_ERR_ARG, "unknown tuning key %d", key);
+    }
+}
+
 // ---- measurement support ------------------------------------------------------------
 extern "C" int qcbf_timing_enable(qcbf_flow* h, int on) {
     CHECK_H(h);

@@ -31,6 +31,7 @@ FP64 = 0
 FP32_MIXED = 1
 
 K_STAGE1, K_STAGE2, K_STAGE3, K_MEASURE = 0, 1, 2, 3
+TUNE_MAP_MODE, TUNE_MAP_WIN = 0, 1
 K_COUNT = 4
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -66,6 +67,8 @@ SIGNATURES = {
     "qcbf_unitarity": (ctypes.c_int, [_vp, _dp, _dp]),
     "qcbf_synchronize": (ctypes.c_int, [_vp]),
     "qcbf_su3_exp": (ctypes.c_int, [_dp, ctypes.c_int, ctypes.c_double, _dp, _ip]),
+    "qcbf_tune": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_long]),
+    "qcbf_get_tune": (ctypes.c_int, [_vp, ctypes.c_int, _lp]),
     "qcbf_timing_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "qcbf_timing_read": (ctypes.c_int, [_vp, _dp, _lp]),
     "qcbf_copy_bench": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_int, _dp]),
@@ -260,6 +263,15 @@ class GradientFlow:
 
     def synchronize(self):
         self._c(load().qcbf_synchronize(self._h))
+
+    # -- launch tuning
+    def tune(self, key, value):
+        self._c(load().qcbf_tune(self._h, int(key), int(value)))
+
+    def get_tune(self, key):
+        v = ctypes.c_long()
+        self._c(load().qcbf_get_tune(self._h, int(key), ctypes.byref(v)))
+        return v.value
 
     # -- measurement support
     def timing(self, on=True):

@@ -136,9 +136,9 @@ int qcbf_measure(qcbf_flow* h, double* plaq, double* E, double* Q);
 int qcbf_flow_measure(qcbf_flow* h, double eps, int n_steps_gf, int n_rk_steps, double* tQE,
                       double* plaq);
 
-/* Number of exponentials since the last call with t = -1/2 tr X^2 > 1 flagged
- * (exp_su3_luscher warns on stderr, src/su3/utils.cpp:153-155): 0 or 1
- * per kernel launch; reading resets the count. */
+/* *flagged = 1 if any exponential since the previous call had an argument with
+ * t = -1/2 tr X^2 > 1 (exp_su3_luscher warns on stderr in that case,
+ * src/su3/utils.cpp:153-155), else 0.  Reading clears the flag. */
 int qcbf_exp_warnings(qcbf_flow* h, int* flagged);
 
 /* max_{links,ij} |(U U^dag - 1)_ij| and max |det U - 1| of the flowing field
@@ -152,6 +152,19 @@ int qcbf_synchronize(qcbf_flow* h);
  * traceless anti-hermitian 3x3 matrices (18 doubles each, row-major re/im).
  * *flagged (may be NULL) is set to 1 if any had t > 1. */
 int qcbf_su3_exp(const double* Z, int n, double coeff, double* out, int* flagged);
+
+/* ---- launch tuning (no counterpart in the reference) --------------------- */
+enum qcbf_tune_key {
+    QCBF_TUNE_MAP_MODE = 0, /* workgroup -> site-tile order of the flow kernels:
+                               0 linear, 1 one contiguous chunk per XCD,
+                               2 XCD chunks inside windows of MAP_WIN workgroups */
+    QCBF_TUNE_MAP_WIN = 1   /* window for mode 2: a multiple of 8, >= 8 */
+};
+/* Set one launch parameter of h; results do not depend on it (only the order in
+ * which workgroups visit sites changes).  QCBF_ERR_ARG for an unknown key or
+ * value. */
+int qcbf_tune(qcbf_flow* h, int key, long value);
+int qcbf_get_tune(const qcbf_flow* h, int key, long* value);
 
 /* ---- measurement support (no counterpart in the reference) -------------- */
 /* When on, every fused-stage and measurement launch on h is bracketed by HIP

@@ -1,0 +1,48 @@
+"""The C++ drop-in (the reference's own GradientFlow / mpi::observables /
+read_ildg_clime signatures over libqcbflow) driven by a caller shaped like the
+reference's src/bin/test.cpp, against the committed oracle trajectory."""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+EXE = os.path.join(ROOT, "ecmc-qcb_amd", "compat", "flow_measure")
+
+
+def test_compat_caller_matches_golden(golden_dir, tmp_path):
+    print("compat flow_measure on the 4^4 fixture", flush=True)
+    d = tmp_path / "conf"
+    d.mkdir()
+    shutil.copy(os.path.join(golden_dir, "conf_4x4x4x4.ildg"), d / "conf")
+    out = subprocess.run([EXE, str(tmp_path), "conf", "4", "21", "1"], capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    with open(os.path.join(golden_dir, "flow_4x4x4x4.json")) as fp:
+        g = json.load(fp)
+    rows = [l.split()[1:] for l in out.stdout.splitlines() if l.startswith("tQE ")]
+    tqe = np.array(rows, dtype=float)
+    assert tqe.shape == (21, 3)
+    assert np.array_equal(tqe[:, 0], np.array([float(v) for v in g["t"]]))
+    E = np.array([float(v) for v in g["E"]])
+    assert np.max(np.abs(tqe[:, 2] - E) / E) < 1e-12
+    assert np.max(np.abs(tqe[:, 1] - np.array([float(v) for v in g["Q"]]))) < 1e-10
+    P = float([l for l in out.stdout.splitlines() if l.startswith("P ")][0].split()[1])
+    assert abs(P - float(g["P"][0])) < 1e-12 * abs(P)
+    # the reference's progress lines and its "t Q E" file (fixed 10 decimals)
+    assert "n = 20, t = 0.40, Q = " in out.stdout
+    topo = (d / "conf_topo.txt").read_text().splitlines()
+    assert len(topo) == 21 and topo[1].split()[0] == "0.0200000000"
+
+
+def test_compat_missing_file_exits_with_status_1(tmp_path):
+    d = tmp_path / "conf"
+    d.mkdir()
+    out = subprocess.run([EXE, str(tmp_path), "missing", "4", "2", "1"], capture_output=True,
+                         text=True, timeout=60)
+    assert out.returncode == 1 and "missing" in out.stderr
