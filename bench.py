@@ -35,6 +35,11 @@ FP64_PEAK_TFLOPS = 78.6
 FLOW_BYTES_PER_SITE_STEP = 4480.0
 FLOW_FLOPS_PER_SITE_STEP = 40452.0
 MEASURE_FLOPS_PER_SITE = 15150.0
+# FP32-mixed form: U as 12 FP32 reals (48 B/link), accumulator 8 FP32 (32 B/link):
+#   4 * (3*2*48 + 4*32) = 1664 B per site per RK3 step; +504 flops for the
+#   third-row reconstruction.
+MIXED_BYTES_PER_SITE_STEP = 1664.0
+MIXED_FLOPS_PER_SITE_STEP = 40956.0
 
 
 def log(msg):
@@ -76,6 +81,23 @@ def cpu_baseline(links, L, eps, steps, threads):
     }
 
 
+def timed_flow(gf, K, warmup):
+    """K bench steps (RK3 step + measurement) on an already filled handle.
+    Returns (seconds, tQE, plaquettes, per-class kernel ms, per-class launches)."""
+    if warmup > 0:
+        gf.flow_measure(warmup + 1, 1)
+    gf.synchronize()
+    gf.timing(True)
+    gf.synchronize()
+    t0 = time.perf_counter()
+    tqe, plaq = gf.flow_measure(K + 1, 1)
+    gf.synchronize()
+    secs = time.perf_counter() - t0
+    ms, launches = gf.timing_read()
+    gf.timing(False)
+    return secs, tqe, plaq, ms, launches
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +109,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-mixed leg")
+    ap.add_argument("--map-mode", type=int, default=0, help="workgroup->tile order (qcbf_tune)")
+    ap.add_argument("--map-win", type=int, default=64)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run the plain copy kernel on this many MiB (traffic calibration)")
     args = ap.parse_args()
@@ -106,23 +131,12 @@ def main():
     log(f"device {name}, {cus} CUs; lattice {L}, V = {V}")
     gf = qcbflow.GradientFlow(L, args.eps)
     gf.hot_start(args.seed)
-
-    # warmup
-    if args.warmup > 0:
-        gf.flow_measure(args.warmup + 1, 1)
-    gf.synchronize()
+    for key, val in ((qcbflow.TUNE_MAP_MODE, args.map_mode), (qcbflow.TUNE_MAP_WIN, args.map_win)):
+        gf.tune(key, val)
 
     # timed region: K RK3 steps, K+1 measurements, stage kernels bracketed by events
     K = args.steps
-    gf.timing(True)
-    gf.synchronize()
-    t0 = time.perf_counter()
-    tqe, plaq = gf.flow_measure(K + 1, 1)
-    gf.synchronize()
-    t1 = time.perf_counter()
-    ms, launches = gf.timing_read()
-    gf.timing(False)
-    secs = t1 - t0
+    secs, tqe, plaq, ms, launches = timed_flow(gf, K, args.warmup)
     value = V * K / secs
     log(f"timed: {K} steps in {secs:.4f} s -> {value:.4e} site-steps/s; "
         f"E(t={tqe[-1, 0]:.2f}) = {tqe[-1, 2]:.6g}, Q = {tqe[-1, 1]:.6g}")
@@ -150,9 +164,42 @@ def main():
         log(f"copy kernel: {args.calibrate_copy_mib} MiB in {copy_ms:.4f} ms -> "
             f"{2 * (args.calibrate_copy_mib << 20) / (copy_ms * 1e-3) / 1e9:.1f} GB/s (read+write)")
 
+    # the FP32-mixed form on the same lattice and cadence
+    mixed = None
+    if not args.no_mixed:
+        links = gf.field() if not args.no_cpu else None
+        gm = qcbflow.GradientFlow(L, args.eps, qcbflow.FP32_MIXED)
+        gm.hot_start(args.seed)
+        gm.tune(qcbflow.TUNE_MAP_MODE, args.map_mode)
+        gm.tune(qcbflow.TUNE_MAP_WIN, args.map_win)
+        msecs, mtqe, mplaq, mms, mn = timed_flow(gm, K, args.warmup)
+        mstage = (mms[0] + mms[1] + mms[2]) / (mn[0] + mn[1] + mn[2])
+        mach = V * MIXED_BYTES_PER_SITE_STEP / 3.0 / (mstage * 1e-3) / 1e9
+        mixed = {
+            "value": V * K / msecs,
+            "vs_fp64": (V * K / msecs) / value,
+            "ms_per_step": msecs / K * 1e3,
+            "stage_avg_ms": mstage,
+            "measure_avg_ms": mms[3] / max(1, mn[3]),
+            "hbm_achieved_gbs": mach,
+            "hbm_frac": mach / HBM_PEAK_GBS,
+            "bytes_per_site_step": MIXED_BYTES_PER_SITE_STEP,
+            "final_t": float(mtqe[-1, 0]),
+            "dE_rel_vs_fp64_at_final_t": float(abs(mtqe[-1, 2] - tqe[-1, 2]) / tqe[-1, 2]),
+            "dQ_vs_fp64_at_final_t": float(abs(mtqe[-1, 1] - tqe[-1, 1])),
+            "note": "same Haar hot start rounded to FP32; accuracy gates on thermalised "
+                    "lattices are in tests/test_gpu_mixed.py",
+        }
+        log(f"fp32-mixed: {mixed['value']:.4e} site-steps/s ({mixed['vs_fp64']:.2f}x fp64), "
+            f"stage {mstage:.4f} ms")
+        gm.close()
+    else:
+        links = None
+
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
-        links = gf.field()
+        if links is None:
+            links = gf.field()
         threads = min(args.cpu_threads, os.cpu_count() or 1)
         log(f"cpu baseline: {args.cpu_steps} steps on {threads} threads ...")
         cpu = cpu_baseline(links, L, args.eps, args.cpu_steps, threads)
@@ -182,6 +229,7 @@ def main():
             "measure_every_rk3_steps": 1,
             "precision": "fp64",
             "parallelism": "single GPU",
+            "tile_map": [args.map_mode, args.map_win],
         },
         "roofline": {
             "bound": "hbm",
@@ -199,6 +247,7 @@ def main():
         },
         "measure_avg_ms": meas_avg_ms,
         "flow_only_value": flow_only,
+        "fp32_mixed": mixed,
         "cpu_baseline": cpu,
     }
     if copy_ms is not None:

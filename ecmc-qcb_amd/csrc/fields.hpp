@@ -1,0 +1,153 @@
+// fields.hpp -- device access to link fields and su(3) fields in HBM, for the
+// two storage forms of the flow.
+//
+// field64  (plain path): 9 complex entries per link, FP64, structure of arrays
+//          U[(mu*9 + k)*V + s] as double2 (lattice.hpp).
+// field32c (FP32-mixed path): the first two rows only, FP32, as three float4
+//          per link, U[(mu*3 + w)*V + s]:
+//            w = 0: (U00, U01)   w = 1: (U02, U10)   w = 2: (U11, U12);
+//          the third row is rebuilt on load as conj(row0 x row1), which is
+//          exact for SU(3) (U^dag = adj U when det U = 1).  Stores
+//          re-unitarise the two rows in FP64 by row Gram-Schmidt (the
+//          reference's proj_su3, src/su3/utils.cpp:194-214, does the same by
+//          columns on full matrices; with two stored rows the row form is the
+//          natural one), so the stored field stays in SU(3) to FP32 rounding.
+// su(3) fields (force / RK3 accumulator) hold the 8 reals of alg8:
+//          FP64 as 4 double2 per link, FP32 as 2 float4 per link.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "lattice.hpp"
+
+namespace qcb {
+
+enum class prec { f64 = 0, f32m = 1 };
+
+// ---- FP32 two-row compressed field -------------------------------------------
+struct field32c {
+    using real = float;
+    static constexpr bool stream_rows = false;
+    const float4* __restrict__ p;
+    unsigned V;
+    __device__ inline mat3<float> link(int mu, unsigned s) const {
+        const float4* b = p + (size_t)mu * 3 * V + s;
+        const float4 a = b[0], c = b[V], d = b[2 * (size_t)V];
+        mat3<float> m;
+        m.e[0][0] = {a.x, a.y};
+        m.e[0][1] = {a.z, a.w};
+        m.e[0][2] = {c.x, c.y};
+        m.e[1][0] = {c.z, c.w};
+        m.e[1][1] = {d.x, d.y};
+        m.e[1][2] = {d.z, d.w};
+        m.e[2][0] = conj(m.e[0][1] * m.e[1][2] - m.e[0][2] * m.e[1][1]);
+        m.e[2][1] = conj(m.e[0][2] * m.e[1][0] - m.e[0][0] * m.e[1][2]);
+        m.e[2][2] = conj(m.e[0][0] * m.e[1][1] - m.e[0][1] * m.e[1][0]);
+        return m;
+    }
+};
+
+// FP64 Gram-Schmidt of two rows: r0 <- r0/|r0|, r1 <- (r1 - <r0,r1> r0)/|.|
+__device__ inline void orthonormalise_rows(cpx<double> (&r0)[3], cpx<double> (&r1)[3]) {
+    double n0 = norm2(r0[0]) + norm2(r0[1]) + norm2(r0[2]);
+    const double i0 = 1.0 / sqrt(n0);
+#pragma unroll
+    for (int k = 0; k < 3; k++) r0[k] = i0 * r0[k];
+    cpx<double> dot = {0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 3; k++) dot = madd_cl(dot, r0[k], r1[k]);
+#pragma unroll
+    for (int k = 0; k < 3; k++) r1[k] = r1[k] - dot * r0[k];
+    const double n1 = norm2(r1[0]) + norm2(r1[1]) + norm2(r1[2]);
+    const double i1 = 1.0 / sqrt(n1);
+#pragma unroll
+    for (int k = 0; k < 3; k++) r1[k] = i1 * r1[k];
+}
+
+__device__ inline void store_rows32c(float4* __restrict__ p, unsigned V, int mu, unsigned s,
+                                     cpx<double> (&r0)[3], cpx<double> (&r1)[3]) {
+    orthonormalise_rows(r0, r1);
+    float4* b = p + (size_t)mu * 3 * V + s;
+    b[0] = make_float4((float)r0[0].r, (float)r0[0].i, (float)r0[1].r, (float)r0[1].i);
+    b[V] = make_float4((float)r0[2].r, (float)r0[2].i, (float)r1[0].r, (float)r1[0].i);
+    b[2 * (size_t)V] = make_float4((float)r1[1].r, (float)r1[1].i, (float)r1[2].r, (float)r1[2].i);
+}
+
+template <typename T>
+__device__ inline void store_link32c(float4* __restrict__ p, unsigned V, int mu, unsigned s,
+                                     const mat3<T>& m) {
+    cpx<double> r0[3], r1[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        r0[k] = {(double)m.e[0][k].r, (double)m.e[0][k].i};
+        r1[k] = {(double)m.e[1][k].r, (double)m.e[1][k].i};
+    }
+    store_rows32c(p, V, mu, s, r0, r1);
+}
+
+__device__ inline alg8<float> load_alg32(const float4* __restrict__ p, unsigned V, int mu,
+                                         unsigned s) {
+    const float4* b = p + (size_t)mu * 2 * V + s;
+    const float4 a = b[0], c = b[V];
+    return alg8<float>{{a.x, a.y, a.z, a.w, c.x, c.y, c.z, c.w}};
+}
+__device__ inline void store_alg32(float4* __restrict__ p, unsigned V, int mu, unsigned s,
+                                   const alg8<float>& a) {
+    float4* b = p + (size_t)mu * 2 * V + s;
+    b[0] = make_float4(a.a[0], a.a[1], a.a[2], a.a[3]);
+    b[V] = make_float4(a.a[4], a.a[5], a.a[6], a.a[7]);
+}
+
+// ---- storage traits used by the generic kernels ------------------------------
+template <prec P>
+struct storage;
+
+template <>
+struct storage<prec::f64> {
+    using real = double;
+    using field = field64;
+    using vec = double2;   // link-field word
+    using avec = double2;  // su(3)-field word
+    static constexpr int link_words = 9;
+    static constexpr int alg_words = 4;
+    __device__ static field view(const vec* p, unsigned V) { return field{p, V}; }
+    __device__ static void store(vec* p, unsigned V, int mu, unsigned s, const mat3<double>& m) {
+        store_link64(p, V, mu, s, m);
+    }
+    __device__ static alg8<double> load_alg(const avec* p, unsigned V, int mu, unsigned s) {
+        return load_alg64(p, V, mu, s);
+    }
+    __device__ static void store_alg(avec* p, unsigned V, int mu, unsigned s,
+                                     const alg8<double>& a) {
+        store_alg64(p, V, mu, s, a);
+    }
+};
+
+template <>
+struct storage<prec::f32m> {
+    using real = float;
+    using field = field32c;
+    using vec = float4;
+    using avec = float4;
+    static constexpr int link_words = 3;
+    static constexpr int alg_words = 2;
+    __device__ static field view(const vec* p, unsigned V) { return field{p, V}; }
+    __device__ static void store(vec* p, unsigned V, int mu, unsigned s, const mat3<float>& m) {
+        store_link32c(p, V, mu, s, m);
+    }
+    __device__ static alg8<float> load_alg(const avec* p, unsigned V, int mu, unsigned s) {
+        return load_alg32(p, V, mu, s);
+    }
+    __device__ static void store_alg(avec* p, unsigned V, int mu, unsigned s,
+                                     const alg8<float>& a) {
+        store_alg32(p, V, mu, s, a);
+    }
+};
+
+inline size_t link_field_bytes(prec p, unsigned V) {
+    return (size_t)V * 4 * (p == prec::f64 ? 9 * sizeof(double2) : 3 * sizeof(float4));
+}
+inline size_t alg_field_bytes(prec p, unsigned V) {
+    return (size_t)V * 4 * (p == prec::f64 ? 4 * sizeof(double2) : 2 * sizeof(float4));
+}
+
+}  // namespace qcb

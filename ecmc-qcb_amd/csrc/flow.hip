@@ -1,0 +1,302 @@
+// flow.hip -- Wilson-flow kernels for gfx950, FP64 and FP32-mixed.
+//
+// The reference runs one RK3 step as seven OpenMP passes plus three halo
+// exchanges (GradientFlow::rk3_step, src/flow/gradient_flow.cpp:124-135).
+// Here each RK3 stage is ONE kernel: for every link (x, mu) it forms the staple
+// sum (GaugeField::compute_staple, src/gauge/GaugeField.cpp:74-97), the force
+// Z = P_TA(-U Sigma) (compute_force, gradient_flow.cpp:17-37), the RK3
+// combination with an 8-real su(3) accumulator (compute_w1/w2/w3 and
+// replace_force_0, gradient_flow.cpp:40-121), the Luscher exponential
+// (exp_su3_luscher, src/su3/utils.cpp:145-193) and the update exp(X) U.
+// Neighbours still need the old links, so the new links go to a second
+// (ping-pong) buffer.
+//
+// The same code serves both storage forms (fields.hpp): the FP64 path does all
+// arithmetic in FP64; the FP32-mixed path does the staples, force, RK3
+// combination and the exp reconstruction in FP32 and keeps FP64 for the
+// Cayley-Hamilton coefficients (t, d and the Horner recursion) and for the
+// re-unitarisation of the stored rows.
+//
+// Work mapping: a workgroup is 64 consecutive sites x 4 directions; wavefront
+// w handles direction mu = w, so every index computation inside a wavefront is
+// uniform and the four wavefronts of one workgroup share neighbour links
+// through the CU's L1.  The workgroup -> site-tile order is XCD-aware
+// (tile_of_block, lattice.hpp).
+#include "fields.hpp"
+#include "kernels.hpp"
+
+namespace qcb {
+
+// Staple sum of U_mu(x):
+//   sum_{nu != mu} U_nu(x+mu) (U_nu(x) U_mu(x+nu))^dag
+//                + (U_mu(x-nu) U_nu(x+mu-nu))^dag U_nu(x-nu)
+// For FP64 one operand of every product is streamed from memory a row at a
+// time, so only the accumulator, one product and one full operand are live;
+// FP32 matrices are half the registers and are loaded whole.
+template <class F, int MU>
+__device__ inline mat3<typename F::real> staple_sum(const F& f, unsigned s, const nbr& n) {
+    using T = typename F::real;
+    mat3<T> st = zero3<T>();
+    const unsigned xmu = s + n.up[MU];
+#pragma unroll
+    for (int nu = 0; nu < 4; nu++) {
+        if (nu == MU) continue;
+        const unsigned xnu = s + n.up[nu];
+        const unsigned xmnu = s + n.dn[nu];
+        const unsigned xmunu = xmu + n.dn[nu];
+        if constexpr (F::stream_rows) {
+            {  // T = U_nu(x) U_mu(x+nu); st += U_nu(x+mu) T^dag
+                const mat3<T> B = f.link(MU, xnu);
+                mat3<T> P;
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    cpx<T> a[3];
+                    f.row(nu, s, i, a);
+#pragma unroll
+                    for (int j = 0; j < 3; j++)
+                        P.e[i][j] = madd(madd(a[0] * B.e[0][j], a[1], B.e[1][j]), a[2], B.e[2][j]);
+                }
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    cpx<T> c[3];
+                    f.row(nu, xmu, i, c);
+#pragma unroll
+                    for (int j = 0; j < 3; j++)
+                        st.e[i][j] = madd_c(madd_c(madd_c(st.e[i][j], c[0], P.e[j][0]), c[1],
+                                                   P.e[j][1]),
+                                            c[2], P.e[j][2]);
+                }
+            }
+            {  // T = U_mu(x-nu) U_nu(x+mu-nu); st += T^dag U_nu(x-nu)
+                const mat3<T> B = f.link(nu, xmunu);
+                mat3<T> P;
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    cpx<T> a[3];
+                    f.row(MU, xmnu, i, a);
+#pragma unroll
+                    for (int j = 0; j < 3; j++)
+                        P.e[i][j] = madd(madd(a[0] * B.e[0][j], a[1], B.e[1][j]), a[2], B.e[2][j]);
+                }
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    cpx<T> v[3];
+                    f.row(nu, xmnu, k, v);  // row k of U_nu(x-nu)
+#pragma unroll
+                    for (int i = 0; i < 3; i++)
+#pragma unroll
+                        for (int j = 0; j < 3; j++) st.e[i][j] = madd_cl(st.e[i][j], P.e[k][i], v[j]);
+                }
+            }
+        } else {
+            {
+                const mat3<T> a = mul(f.link(nu, s), f.link(MU, xnu));
+                acc_mul_nd(st, f.link(nu, xmu), a);
+            }
+            {
+                const mat3<T> b = mul(f.link(MU, xmnu), f.link(nu, xmunu));
+                acc_mul_dn(st, b, f.link(nu, xmnu));
+            }
+        }
+    }
+    return st;
+}
+
+template <class F, int MU>
+__device__ inline alg8<typename F::real> force_at(const F& f, unsigned s, const nbr& n,
+                                                  mat3<typename F::real>& u) {
+    const mat3<typename F::real> st = staple_sum<F, MU>(f, s, n);
+    u = f.link(MU, s);
+    return neg_proj_ta(mul(u, st));
+}
+
+template <prec P, int STAGE, int MU>
+__device__ inline void stage_body(const typename storage<P>::field& f,
+                                  typename storage<P>::vec* __restrict__ Uout,
+                                  typename storage<P>::avec* __restrict__ acc, unsigned s,
+                                  const nbr& n, double cz, double ca, bool& warn) {
+    using S = storage<P>;
+    using T = typename S::real;
+    mat3<T> u;
+    const alg8<T> z = force_at<typename S::field, MU>(f, s, n, u);
+    const T czr = (T)cz, car = (T)ca;
+    alg8<T> x;
+    if constexpr (STAGE == 1) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) x.a[k] = czr * z.a[k];
+        S::store_alg(acc, f.V, MU, s, z);
+    } else {
+        alg8<T> a = S::load_alg(acc, f.V, MU, s);
+#pragma unroll
+        for (int k = 0; k < 8; k++) x.a[k] = czr * z.a[k] - car * a.a[k];
+        if constexpr (STAGE == 2) {
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                a.a[k] = (T)(8.0 / 9.0) * z.a[k] - (T)(17.0 / 36.0) * a.a[k];
+            S::store_alg(acc, f.V, MU, s, a);
+        }
+    }
+    const mat3<T> e = exp_su3<T, double>(x, &warn);
+    S::store(Uout, f.V, MU, s, mul(e, u));
+}
+
+#ifndef QCB_STAGE_WAVES_PER_EU
+#define QCB_STAGE_WAVES_PER_EU 1
+#endif
+
+template <prec P, int STAGE>
+__global__ __launch_bounds__(256, QCB_STAGE_WAVES_PER_EU) void k_rk3_stage(
+    const typename storage<P>::vec* __restrict__ Uin, typename storage<P>::vec* __restrict__ Uout,
+    typename storage<P>::avec* __restrict__ acc, lattice g, double cz, double ca,
+    int* __restrict__ warn) {
+    const unsigned ntiles = (g.V + kSitesPerBlock - 1) / kSitesPerBlock;
+    const unsigned tile = tile_of_block(blockIdx.x, ntiles, g.map_mode, g.map_win);
+    const unsigned s = tile * kSitesPerBlock + threadIdx.x;
+    if (tile >= ntiles || s >= g.V) return;
+    const nbr n = neighbours(g, s);
+    const auto f = storage<P>::view(Uin, g.V);
+    bool w = false;
+    switch (threadIdx.y) {
+        case 0: stage_body<P, STAGE, 0>(f, Uout, acc, s, n, cz, ca, w); break;
+        case 1: stage_body<P, STAGE, 1>(f, Uout, acc, s, n, cz, ca, w); break;
+        case 2: stage_body<P, STAGE, 2>(f, Uout, acc, s, n, cz, ca, w); break;
+        default: stage_body<P, STAGE, 3>(f, Uout, acc, s, n, cz, ca, w); break;
+    }
+    if (w) atomicOr(warn, 1);
+}
+
+template <prec P>
+static hipError_t stage_launch(int stage, const void* Uin, void* Uout, void* acc, const lattice& g,
+                               double cz, double ca, int* warn, hipStream_t st) {
+    using S = storage<P>;
+    const auto* in = static_cast<const typename S::vec*>(Uin);
+    auto* out = static_cast<typename S::vec*>(Uout);
+    auto* a = static_cast<typename S::avec*>(acc);
+    const dim3 block(kSitesPerBlock, 4);
+    const dim3 grid(grid_for_tiles((g.V + kSitesPerBlock - 1) / kSitesPerBlock, g.map_mode,
+                                   g.map_win));
+    switch (stage) {
+        case 1: k_rk3_stage<P, 1><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
+        case 2: k_rk3_stage<P, 2><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
+        case 3: k_rk3_stage<P, 3><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_rk3_stage(prec p, int stage, const void* Uin, void* Uout, void* acc,
+                            const lattice& g, double cz, double ca, int* warn, hipStream_t st) {
+    return p == prec::f64 ? stage_launch<prec::f64>(stage, Uin, Uout, acc, g, cz, ca, warn, st)
+                          : stage_launch<prec::f32m>(stage, Uin, Uout, acc, g, cz, ca, warn, st);
+}
+
+// ---- unfused pieces (GradientFlow member functions) -------------------------
+
+template <prec P, int MU>
+__device__ inline void force_body(const typename storage<P>::field& f,
+                                  typename storage<P>::avec* __restrict__ F, unsigned s,
+                                  const nbr& n) {
+    mat3<typename storage<P>::real> u;
+    storage<P>::store_alg(F, f.V, MU, s, force_at<typename storage<P>::field, MU>(f, s, n, u));
+}
+
+// compute_force: src/flow/gradient_flow.cpp:17-37
+template <prec P>
+__global__ __launch_bounds__(256) void k_force(const typename storage<P>::vec* __restrict__ U,
+                                               typename storage<P>::avec* __restrict__ F,
+                                               lattice g) {
+    const unsigned s = blockIdx.x * kSitesPerBlock + threadIdx.x;
+    if (s >= g.V) return;
+    const nbr n = neighbours(g, s);
+    const auto f = storage<P>::view(U, g.V);
+    switch (threadIdx.y) {
+        case 0: force_body<P, 0>(f, F, s, n); break;
+        case 1: force_body<P, 1>(f, F, s, n); break;
+        case 2: force_body<P, 2>(f, F, s, n); break;
+        default: force_body<P, 3>(f, F, s, n); break;
+    }
+}
+
+// compute_w1 / compute_w2 / compute_w3: src/flow/gradient_flow.cpp:40-79,102-121.
+// One thread per link, in place.
+template <prec P>
+__global__ __launch_bounds__(256) void k_update(int which, typename storage<P>::vec* __restrict__ U,
+                                                const typename storage<P>::avec* __restrict__ F0,
+                                                const typename storage<P>::avec* __restrict__ F1,
+                                                lattice g, double eps, int* __restrict__ warn) {
+    using S = storage<P>;
+    using T = typename S::real;
+    const unsigned s = blockIdx.x * kSitesPerBlock + threadIdx.x;
+    if (s >= g.V) return;
+    const int mu = threadIdx.y;
+    const alg8<T> a0 = S::load_alg(F0, g.V, mu, s);
+    alg8<T> x;
+    if (which == 1) {
+        const T c = (T)(0.25 * eps);
+        for (int k = 0; k < 8; k++) x.a[k] = c * a0.a[k];
+    } else {
+        const alg8<T> a1 = S::load_alg(F1, g.V, mu, s);
+        const T c1 = (T)((which == 2) ? (8.0 / 9.0) * eps : 0.75 * eps);
+        const T c0 = (T)((which == 2) ? (17.0 / 36.0) * eps : eps);
+        for (int k = 0; k < 8; k++) x.a[k] = c1 * a1.a[k] - c0 * a0.a[k];
+    }
+    bool w = false;
+    const mat3<T> e = exp_su3<T, double>(x, &w);
+    const mat3<T> u = S::view(U, g.V).link(mu, s);
+    S::store(U, g.V, mu, s, mul(e, u));
+    if (w) atomicOr(warn, 1);
+}
+
+// replace_force_0: src/flow/gradient_flow.cpp:82-99.  F0 <- 8/9 F1 - 17/36 F0.
+template <prec P>
+__global__ __launch_bounds__(256) void k_replace(typename storage<P>::avec* __restrict__ F0,
+                                                 const typename storage<P>::avec* __restrict__ F1,
+                                                 lattice g) {
+    using S = storage<P>;
+    using T = typename S::real;
+    const unsigned s = blockIdx.x * kSitesPerBlock + threadIdx.x;
+    if (s >= g.V) return;
+    const int mu = threadIdx.y;
+    const alg8<T> a0 = S::load_alg(F0, g.V, mu, s), a1 = S::load_alg(F1, g.V, mu, s);
+    alg8<T> r;
+    for (int k = 0; k < 8; k++) r.a[k] = (T)(8.0 / 9.0) * a1.a[k] - (T)(17.0 / 36.0) * a0.a[k];
+    S::store_alg(F0, g.V, mu, s, r);
+}
+
+template <prec P>
+static hipError_t unfused_launch(int which, void* U, void* F0, const void* F1, const lattice& g,
+                                 double eps, int* warn, hipStream_t st) {
+    using S = storage<P>;
+    const dim3 block(kSitesPerBlock, 4);
+    const dim3 grid((g.V + kSitesPerBlock - 1) / kSitesPerBlock);
+    auto* u = static_cast<typename S::vec*>(U);
+    auto* f0 = static_cast<typename S::avec*>(F0);
+    const auto* f1 = static_cast<const typename S::avec*>(F1);
+    if (which == 0)
+        k_force<P><<<grid, block, 0, st>>>(u, f0, g);  // F0 is the output here
+    else if (which <= 3)
+        k_update<P><<<grid, block, 0, st>>>(which, u, f0, f1, g, eps, warn);
+    else
+        k_replace<P><<<grid, block, 0, st>>>(f0, f1, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_force(prec p, const void* U, void* F, const lattice& g, hipStream_t st) {
+    void* u = const_cast<void*>(U);
+    return p == prec::f64 ? unfused_launch<prec::f64>(0, u, F, nullptr, g, 0.0, nullptr, st)
+                          : unfused_launch<prec::f32m>(0, u, F, nullptr, g, 0.0, nullptr, st);
+}
+hipError_t launch_update(prec p, int which, void* U, const void* F0, const void* F1,
+                         const lattice& g, double eps, int* warn, hipStream_t st) {
+    if (which < 1 || which > 3) return hipErrorInvalidValue;
+    void* f0 = const_cast<void*>(F0);
+    return p == prec::f64 ? unfused_launch<prec::f64>(which, U, f0, F1, g, eps, warn, st)
+                          : unfused_launch<prec::f32m>(which, U, f0, F1, g, eps, warn, st);
+}
+hipError_t launch_replace_force0(prec p, void* F0, const void* F1, const lattice& g,
+                                 hipStream_t st) {
+    return p == prec::f64 ? unfused_launch<prec::f64>(4, nullptr, F0, F1, g, 0.0, nullptr, st)
+                          : unfused_launch<prec::f32m>(4, nullptr, F0, F1, g, 0.0, nullptr, st);
+}
+
+}  // namespace qcb

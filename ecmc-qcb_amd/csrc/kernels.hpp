@@ -1,9 +1,12 @@
 // kernels.hpp -- host-side launchers of the HIP kernels (internal to libqcbflow).
+//
+// Link and su(3) field pointers are untyped here; their layout is fixed by the
+// precision `p` (fields.hpp).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "lattice.hpp"
+#include "fields.hpp"
 
 namespace qcb {
 
@@ -16,50 +19,48 @@ inline unsigned measure_blocks(const lattice& g) {
     return (g.V + kSitesPerBlock - 1) / kSitesPerBlock;
 }
 
-// ---- FP64 flow -------------------------------------------------------------
+// ---- flow ------------------------------------------------------------------------
 // One fused RK3 stage (1, 2 or 3): staples -> -P_TA(U Sigma) -> RK3
 // combination with the su(3) accumulator -> exp -> exp(X) U into Uout.
 //   stage 1: X = cz Z,          acc <- Z
 //   stage 2: X = cz Z - ca acc, acc <- 8/9 Z - 17/36 acc
 //   stage 3: X = cz Z - ca acc
-hipError_t launch_rk3_stage_f64(int stage, const double2* Uin, double2* Uout, double2* acc,
-                                const lattice& g, double cz, double ca, int* warn,
-                                hipStream_t st);
+hipError_t launch_rk3_stage(prec p, int stage, const void* Uin, void* Uout, void* acc,
+                            const lattice& g, double cz, double ca, int* warn, hipStream_t st);
 
 // Unfused pieces, one per GradientFlow member function.
-hipError_t launch_force_f64(const double2* U, double2* F, const lattice& g, hipStream_t st);
+hipError_t launch_force(prec p, const void* U, void* F, const lattice& g, hipStream_t st);
 // which = 1: U <- exp(eps/4 F0) U ; 2: exp(8/9 eps F1 - 17/36 eps F0) U ;
 // 3: exp(3/4 eps F1 - eps F0) U.  In place.
-hipError_t launch_update_f64(int which, double2* U, const double2* F0, const double2* F1,
-                             const lattice& g, double eps, int* warn, hipStream_t st);
-hipError_t launch_replace_force0_f64(double2* F0, const double2* F1, const lattice& g,
-                                     hipStream_t st);
-
-// Measurement: per-workgroup partial sums (plaquette, E, Q-density) written to
-// partials[3*nblocks], then reduced in a fixed order into out[0..2] =
-// {P, E, Q} with the reference normalisations.  clover = false computes the
-// plaquette only (out[1], out[2] = 0).
-hipError_t launch_measure_f64(const double2* U, const lattice& g, double* partials, double* out,
-                              bool clover, hipStream_t st);
-
-// ---- layout conversion and initialisation ----------------------------------
-// AoS (reference / ILDG order: site, mu, i, j, re/im) <-> SoA.  bswap = true
-// byte-swaps every 64-bit word (big-endian ILDG payload).  nsites sites
-// starting at site0 are converted; src/dst AoS buffers hold just those sites.
-hipError_t launch_aos_to_soa_f64(const double* aos, double2* soa, const lattice& g,
-                                 unsigned site0, unsigned nsites, bool bswap, hipStream_t st);
-hipError_t launch_aos32_to_soa_f64(const float* aos, double2* soa, const lattice& g,
-                                   unsigned site0, unsigned nsites, bool bswap, hipStream_t st);
-hipError_t launch_soa_to_aos_f64(const double2* soa, double* aos, const lattice& g,
-                                 unsigned site0, unsigned nsites, bool bswap, hipStream_t st);
-hipError_t launch_alg_to_aos_f64(const double2* alg, double* aos, const lattice& g,
+hipError_t launch_update(prec p, int which, void* U, const void* F0, const void* F1,
+                         const lattice& g, double eps, int* warn, hipStream_t st);
+hipError_t launch_replace_force0(prec p, void* F0, const void* F1, const lattice& g,
                                  hipStream_t st);
-hipError_t launch_hot_start_f64(double2* U, const lattice& g, uint64_t seed, hipStream_t st);
-hipError_t launch_cold_start_f64(double2* U, const lattice& g, hipStream_t st);
+
+// ---- observables -----------------------------------------------------------------
+// Per-workgroup partial sums (plaquette, E, Q-density) written to
+// partials[3*measure_blocks], then reduced in a fixed order into
+// out[0..2] = {P, E, Q} with the reference normalisations.  clover = false
+// computes the plaquette only (out[1], out[2] = 0).
+hipError_t launch_measure(prec p, const void* U, const lattice& g, double* partials, double* out,
+                          bool clover, hipStream_t st);
 // max over links of |U U^dag - 1|_max and of |det U - 1|, into out[0..1]
-// (partials sized measure_blocks(g)*2)
-hipError_t launch_unitarity_f64(const double2* U, const lattice& g, double* partials,
-                                double* out, hipStream_t st);
+// (partials sized 2*measure_blocks)
+hipError_t launch_unitarity(prec p, const void* U, const lattice& g, double* partials,
+                            double* out, hipStream_t st);
+
+// ---- layout conversion and initialisation ----------------------------------------
+// nsites sites starting at site0, 72 words per site in the reference / ILDG
+// order (site, mu, i, j, re/im) <-> the device field.
+//   kind: 0 native doubles, 1 big-endian doubles, 2 big-endian floats.
+hipError_t launch_aos_to_field(prec p, const void* aos, int kind, void* U, const lattice& g,
+                               unsigned site0, unsigned nsites, hipStream_t st);
+hipError_t launch_field_to_aos(prec p, const void* U, double* aos, const lattice& g,
+                               unsigned site0, unsigned nsites, bool bswap, hipStream_t st);
+hipError_t launch_alg_to_aos(prec p, const void* alg, double* aos, const lattice& g,
+                             hipStream_t st);
+hipError_t launch_hot_start(prec p, void* U, const lattice& g, uint64_t seed, hipStream_t st);
+hipError_t launch_cold_start(prec p, void* U, const lattice& g, hipStream_t st);
 
 // exp test entry: n matrices Z (18 doubles each, AoS) -> exp(coeff * P(Z)) where
 // P keeps the traceless anti-hermitian entries Z is assumed to have.

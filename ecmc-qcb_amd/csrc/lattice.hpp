@@ -97,6 +97,8 @@ __device__ inline nbr neighbours(const lattice& g, unsigned s) {
 
 // ---- FP64 structure-of-arrays link field -----------------------------------
 struct field64 {
+    using real = double;
+    static constexpr bool stream_rows = true;
     const double2* __restrict__ p;
     unsigned V;
     __device__ inline mat3<double> link(int mu, unsigned s) const {

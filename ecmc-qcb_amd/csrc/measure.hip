@@ -1,4 +1,4 @@
-// measure_f64.hip -- fused clover / plaquette / Q / E measurement (FP64).
+// measure.hip -- fused clover / plaquette / Q / E measurement (FP64 and FP32-mixed).
 //
 // Replaces mean_plaquette_local (src/observables/observables_mpi.cpp:11-35),
 // clover_site (:66-90), local_q_e_clover (:93-112) and topo_q_e_clover[_global]
@@ -6,10 +6,14 @@
 // (mu,nu) is exactly the plaquette product of mean_plaquette_local, so the
 // plaquette comes for free.
 //
+// The FP32-mixed path forms the clovers in FP32 and accumulates every
+// per-site contribution in FP64; the final reduction is Neumaier-compensated.
+//
 // Work mapping: a workgroup is 64 consecutive sites x 3 plane pairs.  Pair 0
 // handles planes (0,1),(2,3), pair 1 (0,2),(1,3), pair 2 (0,3),(1,2), so each
 // thread owns one term of q = Re tr F01F23 - Re tr F02F13 + Re tr F03F12 and
 // two of the six energy terms and plaquettes.
+#include "fields.hpp"
 #include "kernels.hpp"
 #include "reduce.hpp"
 
@@ -17,100 +21,108 @@ namespace qcb {
 
 // Clover field strength F = (C - C^dag)/8 (trace kept, as the reference does):
 // anti-hermitian, stored as the imaginary diagonal and the upper triangle.
+template <typename T>
 struct fclov {
-    double d[3];
-    cpx<double> o[3];  // (0,1), (0,2), (1,2)
+    T d[3];
+    cpx<T> o[3];  // (0,1), (0,2), (1,2)
 };
 
 // Sum of the four leaves of plane (MU,NU) at x, all anticlockwise, in the
 // order and association of clover_site (observables_mpi.cpp:78-86).  Adds
 // Re tr(leaf 0) to plaq.
-template <int MU, int NU>
-__device__ inline fclov clover64(const field64& f, unsigned s, const nbr& n, double& plaq) {
+template <class F, int MU, int NU>
+__device__ inline fclov<typename F::real> clover(const F& f, unsigned s, const nbr& n,
+                                                 double& plaq) {
+    using T = typename F::real;
     const unsigned xpmu = s + n.up[MU], xpnu = s + n.up[NU];
     const unsigned xmmu = s + n.dn[MU], xmnu = s + n.dn[NU];
     const unsigned xmmupnu = xmmu + n.up[NU], xmmumnu = xmmu + n.dn[NU];
     const unsigned xpmumnu = xpmu + n.dn[NU];
-    mat3<double> C;
+    mat3<T> C;
     {  // U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
-        const mat3<double> a = mul(f.link(MU, s), f.link(NU, xpmu));
-        const mat3<double> b = mul_nd(a, f.link(MU, xpnu));
+        const mat3<T> a = mul(f.link(MU, s), f.link(NU, xpmu));
+        const mat3<T> b = mul_nd(a, f.link(MU, xpnu));
         C = mul_nd(b, f.link(NU, s));
-        plaq += C.e[0][0].r + C.e[1][1].r + C.e[2][2].r;
+        plaq += (double)(C.e[0][0].r + C.e[1][1].r + C.e[2][2].r);
     }
     {  // U_nu(x) U_mu(x-mu+nu)^dag U_nu(x-mu)^dag U_mu(x-mu)
-        const mat3<double> a = mul_nd(f.link(NU, s), f.link(MU, xmmupnu));
-        const mat3<double> b = mul_nd(a, f.link(NU, xmmu));
+        const mat3<T> a = mul_nd(f.link(NU, s), f.link(MU, xmmupnu));
+        const mat3<T> b = mul_nd(a, f.link(NU, xmmu));
         acc_mul(C, b, f.link(MU, xmmu));
     }
     {  // U_mu(x-mu)^dag U_nu(x-mu-nu)^dag U_mu(x-mu-nu) U_nu(x-nu)
-        const mat3<double> a = mul(f.link(NU, xmmumnu), f.link(MU, xmmu));
-        const mat3<double> b = mul(f.link(MU, xmmumnu), f.link(NU, xmnu));
+        const mat3<T> a = mul(f.link(NU, xmmumnu), f.link(MU, xmmu));
+        const mat3<T> b = mul(f.link(MU, xmmumnu), f.link(NU, xmnu));
         acc_mul_dn(C, a, b);
     }
     {  // U_nu(x-nu)^dag U_mu(x-nu) U_nu(x+mu-nu) U_mu(x)^dag
-        const mat3<double> a = mul_dn(f.link(NU, xmnu), f.link(MU, xmnu));
-        const mat3<double> b = mul(a, f.link(NU, xpmumnu));
+        const mat3<T> a = mul_dn(f.link(NU, xmnu), f.link(MU, xmnu));
+        const mat3<T> b = mul(a, f.link(NU, xpmumnu));
         acc_mul_nd(C, b, f.link(MU, s));
     }
-    fclov F;
+    fclov<T> Fc;
+    const T e8 = (T)0.125;
 #pragma unroll
-    for (int k = 0; k < 3; k++) F.d[k] = (C.e[k][k].i + C.e[k][k].i) * 0.125;
-    F.o[0] = 0.125 * (C.e[0][1] - conj(C.e[1][0]));
-    F.o[1] = 0.125 * (C.e[0][2] - conj(C.e[2][0]));
-    F.o[2] = 0.125 * (C.e[1][2] - conj(C.e[2][1]));
-    return F;
+    for (int k = 0; k < 3; k++) Fc.d[k] = (C.e[k][k].i + C.e[k][k].i) * e8;
+    Fc.o[0] = e8 * (C.e[0][1] - conj(C.e[1][0]));
+    Fc.o[1] = e8 * (C.e[0][2] - conj(C.e[2][0]));
+    Fc.o[2] = e8 * (C.e[1][2] - conj(C.e[2][1]));
+    return Fc;
 }
 
-template <int MU, int NU>
-__device__ inline void plaq64(const field64& f, unsigned s, const nbr& n, double& plaq) {
+template <class F, int MU, int NU>
+__device__ inline void plaq_only(const F& f, unsigned s, const nbr& n, double& plaq) {
+    using T = typename F::real;
     const unsigned xpmu = s + n.up[MU], xpnu = s + n.up[NU];
-    const mat3<double> a = mul(f.link(MU, s), f.link(NU, xpmu));
-    const mat3<double> b = mul_nd(a, f.link(MU, xpnu));
-    plaq += re_tr_nd(b, f.link(NU, s));
+    const mat3<T> a = mul(f.link(MU, s), f.link(NU, xpmu));
+    const mat3<T> b = mul_nd(a, f.link(MU, xpnu));
+    plaq += (double)re_tr_nd(b, f.link(NU, s));
 }
 
 // |F|_F^2 for anti-hermitian F
-__device__ inline double fnorm2(const fclov& F) {
+template <typename T>
+__device__ inline T fnorm2(const fclov<T>& F) {
     return F.d[0] * F.d[0] + F.d[1] * F.d[1] + F.d[2] * F.d[2] +
-           2.0 * (norm2(F.o[0]) + norm2(F.o[1]) + norm2(F.o[2]));
+           T(2) * (norm2(F.o[0]) + norm2(F.o[1]) + norm2(F.o[2]));
 }
 // Re tr(A B) for anti-hermitian A, B
-__device__ inline double re_tr_ab(const fclov& A, const fclov& B) {
-    double s = A.d[0] * B.d[0] + A.d[1] * B.d[1] + A.d[2] * B.d[2];
-    double o = 0.0;
+template <typename T>
+__device__ inline T re_tr_ab(const fclov<T>& A, const fclov<T>& B) {
+    T s = A.d[0] * B.d[0] + A.d[1] * B.d[1] + A.d[2] * B.d[2];
+    T o = T(0);
 #pragma unroll
     for (int k = 0; k < 3; k++) o += A.o[k].r * B.o[k].r + A.o[k].i * B.o[k].i;
-    return -(s + 2.0 * o);
+    return -(s + T(2) * o);
 }
 
-template <int M1, int N1, int M2, int N2, int SIGN, bool CLOVER>
-__device__ inline void pair_body64(const field64& f, unsigned s, const nbr& n, double& vp,
-                                   double& ve, double& vq) {
+template <class F, int M1, int N1, int M2, int N2, int SIGN, bool CLOVER>
+__device__ inline void pair_body(const F& f, unsigned s, const nbr& n, double& vp, double& ve,
+                                 double& vq) {
     if constexpr (CLOVER) {
-        const fclov A = clover64<M1, N1>(f, s, n, vp);
-        const fclov B = clover64<M2, N2>(f, s, n, vp);
-        ve += 0.5 * (fnorm2(A) + fnorm2(B));
-        vq += SIGN * re_tr_ab(A, B);
+        const auto A = clover<F, M1, N1>(f, s, n, vp);
+        const auto B = clover<F, M2, N2>(f, s, n, vp);
+        ve += 0.5 * ((double)fnorm2(A) + (double)fnorm2(B));
+        vq += SIGN * (double)re_tr_ab(A, B);
     } else {
-        plaq64<M1, N1>(f, s, n, vp);
-        plaq64<M2, N2>(f, s, n, vp);
+        plaq_only<F, M1, N1>(f, s, n, vp);
+        plaq_only<F, M2, N2>(f, s, n, vp);
     }
 }
 
-template <bool CLOVER>
-__global__ __launch_bounds__(192) void k_measure64(const double2* __restrict__ U, lattice g,
-                                                   double* __restrict__ partials) {
+template <prec P, bool CLOVER>
+__global__ __launch_bounds__(192) void k_measure(const typename storage<P>::vec* __restrict__ U,
+                                                 lattice g, double* __restrict__ partials) {
+    using FT = typename storage<P>::field;
     __shared__ double lds[3 * 3];
     const unsigned s = blockIdx.x * kSitesPerBlock + threadIdx.x;
     double v[3] = {0.0, 0.0, 0.0};
     if (s < g.V) {
         const nbr n = neighbours(g, s);
-        const field64 f{U, g.V};
+        const FT f = storage<P>::view(U, g.V);
         switch (threadIdx.y) {
-            case 0: pair_body64<0, 1, 2, 3, +1, CLOVER>(f, s, n, v[0], v[1], v[2]); break;
-            case 1: pair_body64<0, 2, 1, 3, -1, CLOVER>(f, s, n, v[0], v[1], v[2]); break;
-            default: pair_body64<0, 3, 1, 2, +1, CLOVER>(f, s, n, v[0], v[1], v[2]); break;
+            case 0: pair_body<FT, 0, 1, 2, 3, +1, CLOVER>(f, s, n, v[0], v[1], v[2]); break;
+            case 1: pair_body<FT, 0, 2, 1, 3, -1, CLOVER>(f, s, n, v[0], v[1], v[2]); break;
+            default: pair_body<FT, 0, 3, 1, 2, +1, CLOVER>(f, s, n, v[0], v[1], v[2]); break;
         }
     }
     block_sum<3>(v, lds, 3);
@@ -121,19 +133,27 @@ __global__ __launch_bounds__(192) void k_measure64(const double2* __restrict__ U
     }
 }
 
-// Ordered reduction of the per-workgroup partials and the reference's
-// normalisations: P = sum Re tr / 3 / (6V) (observables_mpi.cpp:48,67),
-// E = sum e / V (:173-176), Q = sum q / (4 pi^2) (:132).
+// Ordered, Neumaier-compensated reduction of the per-workgroup partials and
+// the reference's normalisations: P = sum Re tr / 3 / (6V)
+// (observables_mpi.cpp:48,67), E = sum e / V (:173-176), Q = sum q / (4 pi^2)
+// (:132).
+__device__ inline void neumaier_add(double& sum, double& comp, double x) {
+    const double t = sum + x;
+    comp += (fabs(sum) >= fabs(x)) ? (sum - t) + x : (x - t) + sum;
+    sum = t;
+}
+
 __global__ __launch_bounds__(256) void k_reduce_measure(const double* __restrict__ partials,
                                                         unsigned nblk, double invV,
                                                         double* __restrict__ out) {
     __shared__ double lds[4 * 3];
-    double v[3] = {0.0, 0.0, 0.0};
+    double v[3] = {0.0, 0.0, 0.0}, c[3] = {0.0, 0.0, 0.0};
     for (unsigned b = threadIdx.x; b < nblk; b += blockDim.x) {
-        v[0] += partials[3 * b + 0];
-        v[1] += partials[3 * b + 1];
-        v[2] += partials[3 * b + 2];
+#pragma unroll
+        for (int k = 0; k < 3; k++) neumaier_add(v[k], c[k], partials[3 * b + k]);
     }
+#pragma unroll
+    for (int k = 0; k < 3; k++) v[k] += c[k];
     block_sum<3>(v, lds, 4);
     if (threadIdx.x == 0) {
         out[0] = v[0] / 3.0 / 6.0 * invV;
@@ -142,27 +162,38 @@ __global__ __launch_bounds__(256) void k_reduce_measure(const double* __restrict
     }
 }
 
-hipError_t launch_measure_f64(const double2* U, const lattice& g, double* partials, double* out,
-                              bool clover, hipStream_t st) {
+template <prec P>
+static hipError_t measure_launch(const void* U, const lattice& g, double* partials, double* out,
+                                 bool clover, hipStream_t st) {
     const unsigned nblk = measure_blocks(g);
     const dim3 block(kSitesPerBlock, 3);
+    const auto* u = static_cast<const typename storage<P>::vec*>(U);
     if (clover)
-        k_measure64<true><<<nblk, block, 0, st>>>(U, g, partials);
+        k_measure<P, true><<<nblk, block, 0, st>>>(u, g, partials);
     else
-        k_measure64<false><<<nblk, block, 0, st>>>(U, g, partials);
+        k_measure<P, false><<<nblk, block, 0, st>>>(u, g, partials);
     k_reduce_measure<<<1, 256, 0, st>>>(partials, nblk, 1.0 / (double)g.V, out);
     return hipGetLastError();
 }
 
+hipError_t launch_measure(prec p, const void* U, const lattice& g, double* partials, double* out,
+                          bool clover, hipStream_t st) {
+    return p == prec::f64 ? measure_launch<prec::f64>(U, g, partials, out, clover, st)
+                          : measure_launch<prec::f32m>(U, g, partials, out, clover, st);
+}
+
 // ---- unitarity check --------------------------------------------------------
-__global__ __launch_bounds__(256) void k_unitarity64(const double2* __restrict__ U, lattice g,
-                                                     double* __restrict__ partials) {
+template <prec P>
+__global__ __launch_bounds__(256) void k_unitarity(const typename storage<P>::vec* __restrict__ U,
+                                                   lattice g, double* __restrict__ partials) {
     __shared__ double lds[4 * 2];
     const unsigned s = blockIdx.x * kSitesPerBlock + threadIdx.x;
     double um = 0.0, dm = 0.0;
     if (s < g.V) {
-        const field64 f{U, g.V};
-        const mat3<double> u = f.link(threadIdx.y, s);
+        const mat3<typename storage<P>::real> uu = storage<P>::view(U, g.V).link(threadIdx.y, s);
+        mat3<double> u;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) u.e[i][j] = {(double)uu.e[i][j].r, (double)uu.e[i][j].i};
         const mat3<double> p = mul_nd(u, u);
         for (int i = 0; i < 3; i++)
             for (int j = 0; j < 3; j++) {
@@ -220,10 +251,15 @@ __global__ __launch_bounds__(256) void k_reduce_max2(const double* __restrict__ 
     }
 }
 
-hipError_t launch_unitarity_f64(const double2* U, const lattice& g, double* partials,
-                                double* out, hipStream_t st) {
+hipError_t launch_unitarity(prec p, const void* U, const lattice& g, double* partials,
+                            double* out, hipStream_t st) {
     const unsigned nblk = measure_blocks(g);
-    k_unitarity64<<<nblk, dim3(kSitesPerBlock, 4), 0, st>>>(U, g, partials);
+    if (p == prec::f64)
+        k_unitarity<prec::f64><<<nblk, dim3(kSitesPerBlock, 4), 0, st>>>(
+            static_cast<const double2*>(U), g, partials);
+    else
+        k_unitarity<prec::f32m><<<nblk, dim3(kSitesPerBlock, 4), 0, st>>>(
+            static_cast<const float4*>(U), g, partials);
     k_reduce_max2<<<1, 256, 0, st>>>(partials, nblk, out);
     return hipGetLastError();
 }

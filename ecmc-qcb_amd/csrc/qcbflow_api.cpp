@@ -11,6 +11,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -28,16 +29,18 @@ struct qcbf_event {
 struct qcbf_flow {
     int dev = 0;
     int precision = QCBF_FP64;
+    qcb::prec prec = qcb::prec::f64;
     lattice g{};
     hipStream_t stream = nullptr;
-    double2* U[2] = {nullptr, nullptr};  // ping-pong link fields (SoA)
-    int cur = 0;                         // U[cur] is the flowing field
-    double2* acc = nullptr;              // RK3 su(3) accumulator (8 reals / link)
-    double2* F[2] = {nullptr, nullptr};  // force_0, force_1 of the unfused API
-    double* partials = nullptr;          // per-workgroup partial sums
-    double* dres = nullptr;              // device results of measurements
-    size_t dres_cap = 0;                 // in doubles
-    int* dwarn = nullptr;                // exp t > 1 flag
+    void* U[2] = {nullptr, nullptr};  // ping-pong link fields
+    int cur = 0;                      // U[cur] is the flowing field
+    void* acc = nullptr;              // RK3 su(3) accumulator (8 reals / link)
+    void* F[2] = {nullptr, nullptr};  // force_0, force_1 of the unfused API
+    double* partials = nullptr;       // per-workgroup partial sums
+    double* dres = nullptr;           // device results of measurements
+    size_t dres_cap = 0;              // in doubles
+    int* dwarn = nullptr;             // exp t > 1 flag
+    void* stage = nullptr;            // device staging chunk for host <-> HBM transfers
     std::string err;
     // timing
     bool timing = false;
@@ -46,6 +49,10 @@ struct qcbf_flow {
     double tot_ms[QCBF_K_COUNT] = {0, 0, 0, 0};
     long cnt[QCBF_K_COUNT] = {0, 0, 0, 0};
 };
+
+// Sites per host <-> HBM transfer chunk (37.7 MB of FP64 links).
+constexpr unsigned kChunkSites = 65536;
+constexpr size_t kChunkBytes = (size_t)kChunkSites * 72 * sizeof(double);
 
 static thread_local std::string g_err;
 
@@ -122,6 +129,7 @@ static void release(qcbf_flow* h) {
     (void)hipFree(h->partials);
     (void)hipFree(h->dres);
     (void)hipFree(h->dwarn);
+    (void)hipFree(h->stage);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -129,8 +137,6 @@ static void release(qcbf_flow* h) {
 extern "C" const char* qcbf_last_error(const qcbf_flow* h) {
     return h ? h->err.c_str() : g_err.c_str();
 }
-
-static size_t link_field_bytes(const lattice& g) { return (size_t)g.V * 36 * sizeof(double2); }
 
 static int ensure_results(qcbf_flow* h, size_t ndoubles) {
     if (ndoubles <= h->dres_cap) return QCBF_OK;
@@ -146,8 +152,8 @@ static int ensure_results(qcbf_flow* h, size_t ndoubles) {
 extern "C" int qcbf_create(const int dims[4], int precision, qcbf_flow** out) {
     if (!dims || !out) return fail(nullptr, QCBF_ERR_ARG, "null argument");
     *out = nullptr;
-    if (precision != QCBF_FP64)
-        return fail(nullptr, QCBF_ERR_ARG, "precision %d not supported by this build", precision);
+    if (precision != QCBF_FP64 && precision != QCBF_FP32_MIXED)
+        return fail(nullptr, QCBF_ERR_ARG, "unknown precision %d", precision);
     unsigned long long V = 1;
     for (int d = 0; d < 4; d++) {
         if (dims[d] < 2 || dims[d] > 4096)
@@ -157,6 +163,7 @@ extern "C" int qcbf_create(const int dims[4], int precision, qcbf_flow** out) {
     if (V >= (1ull << 28)) return fail(nullptr, QCBF_ERR_ARG, "volume %llu too large", V);
     qcbf_flow* h = new qcbf_flow();
     h->precision = precision;
+    h->prec = precision == QCBF_FP64 ? qcb::prec::f64 : qcb::prec::f32m;
     h->g = qcb::make_lattice(dims);
     hipError_t e = hipGetDevice(&h->dev);
     if (e != hipSuccess) {
@@ -173,15 +180,15 @@ extern "C" int qcbf_create(const int dims[4], int precision, qcbf_flow** out) {
         }                                                                                     \
     } while (0)
     CREATE_CHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-    CREATE_CHK(hipMalloc(&h->U[0], link_field_bytes(h->g)));
-    CREATE_CHK(hipMalloc(&h->U[1], link_field_bytes(h->g)));
-    CREATE_CHK(hipMalloc(&h->acc, (size_t)h->g.V * 16 * sizeof(double2)));
+    CREATE_CHK(hipMalloc(&h->U[0], qcb::link_field_bytes(h->prec, h->g.V)));
+    CREATE_CHK(hipMalloc(&h->U[1], qcb::link_field_bytes(h->prec, h->g.V)));
+    CREATE_CHK(hipMalloc(&h->acc, qcb::alg_field_bytes(h->prec, h->g.V)));
     CREATE_CHK(hipMalloc(&h->partials, (size_t)qcb::measure_blocks(h->g) * 3 * sizeof(double)));
     CREATE_CHK(hipMalloc(&h->dwarn, sizeof(int)));
     CREATE_CHK(hipMemsetAsync(h->dwarn, 0, sizeof(int), h->stream));
     CREATE_CHK(hipMalloc(&h->dres, 64 * sizeof(double)));
     h->dres_cap = 64;
-    if (qcb::launch_cold_start_f64(h->U[0], h->g, h->stream) != hipSuccess) {
+    if (qcb::launch_cold_start(h->prec, h->U[0], h->g, h->stream) != hipSuccess) {
         release(h);
         return fail(nullptr, QCBF_ERR_HIP, "cold start launch failed");
     }
@@ -207,13 +214,23 @@ extern "C" int qcbf_get_dims(const qcbf_flow* h, int dims[4], int* precision) {
 }
 
 // ---- links in / out ------------------------------------------------------------
+static int ensure_stage(qcbf_flow* h) {
+    if (!h->stage) HIPCHK(h, hipMalloc(&h->stage, kChunkBytes));
+    return QCBF_OK;
+}
+
 extern "C" int qcbf_set_links(qcbf_flow* h, const double* links) {
     CHECK_H(h);
     if (!links) return fail(h, QCBF_ERR_ARG, "null links");
-    double* staging = reinterpret_cast<double*>(h->U[h->cur ^ 1]);
-    HIPCHK(h, hipMemcpyAsync(staging, links, (size_t)h->g.V * 72 * sizeof(double),
-                             hipMemcpyHostToDevice, h->stream));
-    HIPCHK(h, qcb::launch_aos_to_soa_f64(staging, h->U[h->cur], h->g, 0, h->g.V, false, h->stream));
+    int rc = ensure_stage(h);
+    if (rc) return rc;
+    for (unsigned s0 = 0; s0 < h->g.V; s0 += kChunkSites) {
+        const unsigned n = std::min(kChunkSites, h->g.V - s0);
+        HIPCHK(h, hipMemcpyAsync(h->stage, links + (size_t)s0 * 72, (size_t)n * 72 * sizeof(double),
+                                 hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, qcb::launch_aos_to_field(h->prec, h->stage, 0, h->U[h->cur], h->g, s0, n,
+                                           h->stream));
+    }
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return QCBF_OK;
 }
@@ -221,23 +238,29 @@ extern "C" int qcbf_set_links(qcbf_flow* h, const double* links) {
 extern "C" int qcbf_get_links(qcbf_flow* h, double* links) {
     CHECK_H(h);
     if (!links) return fail(h, QCBF_ERR_ARG, "null links");
-    double* staging = reinterpret_cast<double*>(h->U[h->cur ^ 1]);
-    HIPCHK(h, qcb::launch_soa_to_aos_f64(h->U[h->cur], staging, h->g, 0, h->g.V, false, h->stream));
-    HIPCHK(h, hipMemcpyAsync(links, staging, (size_t)h->g.V * 72 * sizeof(double),
-                             hipMemcpyDeviceToHost, h->stream));
+    int rc = ensure_stage(h);
+    if (rc) return rc;
+    double* st = static_cast<double*>(h->stage);
+    for (unsigned s0 = 0; s0 < h->g.V; s0 += kChunkSites) {
+        const unsigned n = std::min(kChunkSites, h->g.V - s0);
+        HIPCHK(h, qcb::launch_field_to_aos(h->prec, h->U[h->cur], st, h->g, s0, n, false,
+                                           h->stream));
+        HIPCHK(h, hipMemcpyAsync(links + (size_t)s0 * 72, st, (size_t)n * 72 * sizeof(double),
+                                 hipMemcpyDeviceToHost, h->stream));
+    }
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return QCBF_OK;
 }
 
 extern "C" int qcbf_hot_start(qcbf_flow* h, uint64_t seed) {
     CHECK_H(h);
-    HIPCHK(h, qcb::launch_hot_start_f64(h->U[h->cur], h->g, seed, h->stream));
+    HIPCHK(h, qcb::launch_hot_start(h->prec, h->U[h->cur], h->g, seed, h->stream));
     return QCBF_OK;
 }
 
 extern "C" int qcbf_cold_start(qcbf_flow* h) {
     CHECK_H(h);
-    HIPCHK(h, qcb::launch_cold_start_f64(h->U[h->cur], h->g, h->stream));
+    HIPCHK(h, qcb::launch_cold_start(h->prec, h->U[h->cur], h->g, h->stream));
     return QCBF_OK;
 }
 
@@ -262,7 +285,6 @@ struct file_closer {
         if (fp) fclose(fp);
     }
 };
-constexpr size_t kIoChunk = 64ull << 20;
 }  // namespace
 
 extern "C" int qcbf_load_ildg(qcbf_flow* h, const char* path) {
@@ -290,34 +312,33 @@ extern "C" int qcbf_load_ildg(qcbf_flow* h, const char* path) {
     if (info.data_bytes != expect)
         return fail(h, QCBF_ERR_FORMAT, "%s: binary size %llu bytes, expected %llu", path,
                     info.data_bytes, expect);
-    // Stream the payload through two pinned buffers into the staging field.
-    char* staging = reinterpret_cast<char*>(h->U[h->cur ^ 1]);
+    // Stream the payload through two pinned buffers and a device staging chunk;
+    // each chunk is byte-swapped and transposed into the field on the device.
+    int rc2 = ensure_stage(h);
+    if (rc2) return rc2;
+    const size_t site_bytes = 72 * word;
+    const unsigned chunk_sites = (unsigned)(kChunkBytes / site_bytes);
     pinned_pair pp;
     for (int k = 0; k < 2; k++) {
-        HIPCHK(h, hipHostMalloc(&pp.p[k], kIoChunk, hipHostMallocDefault));
+        HIPCHK(h, hipHostMalloc(&pp.p[k], kChunkBytes, hipHostMallocDefault));
         HIPCHK(h, hipEventCreateWithFlags(&pp.ev[k], hipEventDisableTiming));
     }
     if (fseeko(fp, info.data_offset, SEEK_SET) != 0)
         return fail(h, QCBF_ERR_IO, "%s: seek failed", path);
-    size_t done = 0;
     int k = 0;
     bool used[2] = {false, false};
-    while (done < expect) {
-        const size_t n = (expect - done < kIoChunk) ? (size_t)(expect - done) : kIoChunk;
+    for (unsigned s0 = 0; s0 < h->g.V; s0 += chunk_sites) {
+        const unsigned n = std::min(chunk_sites, h->g.V - s0);
+        const size_t nb = (size_t)n * site_bytes;
         if (used[k]) HIPCHK(h, hipEventSynchronize(pp.ev[k]));
-        if (fread(pp.p[k], 1, n, fp) != n) return fail(h, QCBF_ERR_IO, "%s: short read", path);
-        HIPCHK(h, hipMemcpyAsync(staging + done, pp.p[k], n, hipMemcpyHostToDevice, h->stream));
+        if (fread(pp.p[k], 1, nb, fp) != nb) return fail(h, QCBF_ERR_IO, "%s: short read", path);
+        HIPCHK(h, hipMemcpyAsync(h->stage, pp.p[k], nb, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, qcb::launch_aos_to_field(h->prec, h->stage, word == 8 ? 1 : 2, h->U[h->cur], h->g,
+                                           s0, n, h->stream));
         HIPCHK(h, hipEventRecord(pp.ev[k], h->stream));
         used[k] = true;
-        done += n;
         k ^= 1;
     }
-    if (word == 8)
-        HIPCHK(h, qcb::launch_aos_to_soa_f64(reinterpret_cast<double*>(staging), h->U[h->cur], h->g,
-                                             0, h->g.V, true, h->stream));
-    else
-        HIPCHK(h, qcb::launch_aos32_to_soa_f64(reinterpret_cast<float*>(staging), h->U[h->cur],
-                                               h->g, 0, h->g.V, true, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return QCBF_OK;
 }
@@ -325,8 +346,8 @@ extern "C" int qcbf_load_ildg(qcbf_flow* h, const char* path) {
 extern "C" int qcbf_save_ildg(qcbf_flow* h, const char* path) {
     CHECK_H(h);
     if (!path) return fail(h, QCBF_ERR_ARG, "null path");
-    double* staging = reinterpret_cast<double*>(h->U[h->cur ^ 1]);
-    HIPCHK(h, qcb::launch_soa_to_aos_f64(h->U[h->cur], staging, h->g, 0, h->g.V, true, h->stream));
+    int rc = ensure_stage(h);
+    if (rc) return rc;
     FILE* fp = fopen(path, "wb");
     if (!fp) return fail(h, QCBF_ERR_IO, "cannot open '%s' for writing", path);
     file_closer fc{fp};
@@ -340,15 +361,15 @@ extern "C" int qcbf_save_ildg(qcbf_flow* h, const char* path) {
     ok = ok && qcb::lime_write_header(fp, "ildg-binary-data", nbin, false, true);
     if (!ok) return fail(h, QCBF_ERR_IO, "%s: short write of LIME headers", path);
     pinned_pair pp;
-    for (int k = 0; k < 2; k++) HIPCHK(h, hipHostMalloc(&pp.p[k], kIoChunk, hipHostMallocDefault));
-    const char* src = reinterpret_cast<const char*>(staging);
-    size_t done = 0;
-    while (done < nbin) {
-        const size_t n = (nbin - done < kIoChunk) ? (size_t)(nbin - done) : kIoChunk;
-        HIPCHK(h, hipMemcpyAsync(pp.p[0], src + done, n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipHostMalloc(&pp.p[0], kChunkBytes, hipHostMallocDefault));
+    double* st = static_cast<double*>(h->stage);
+    for (unsigned s0 = 0; s0 < h->g.V; s0 += kChunkSites) {
+        const unsigned n = std::min(kChunkSites, h->g.V - s0);
+        const size_t nb = (size_t)n * 72 * sizeof(double);
+        HIPCHK(h, qcb::launch_field_to_aos(h->prec, h->U[h->cur], st, h->g, s0, n, true, h->stream));
+        HIPCHK(h, hipMemcpyAsync(pp.p[0], st, nb, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));
-        if (fwrite(pp.p[0], 1, n, fp) != n) return fail(h, QCBF_ERR_IO, "%s: short write", path);
-        done += n;
+        if (fwrite(pp.p[0], 1, nb, fp) != nb) return fail(h, QCBF_ERR_IO, "%s: short write", path);
     }
     if (fflush(fp) != 0) return fail(h, QCBF_ERR_IO, "%s: flush failed", path);
     return QCBF_OK;
@@ -362,8 +383,8 @@ static int rk3_once(qcbf_flow* h, double eps) {
         qcbf_event* ev;
         int rc = timing_begin(h, QCBF_K_STAGE1 + stage - 1, &ev);
         if (rc) return rc;
-        HIPCHK(h, qcb::launch_rk3_stage_f64(stage, h->U[h->cur], h->U[h->cur ^ 1], h->acc, h->g,
-                                            cz[stage - 1], ca[stage - 1], h->dwarn, h->stream));
+        HIPCHK(h, qcb::launch_rk3_stage(h->prec, stage, h->U[h->cur], h->U[h->cur ^ 1], h->acc,
+                                        h->g, cz[stage - 1], ca[stage - 1], h->dwarn, h->stream));
         rc = timing_end(h, ev);
         if (rc) return rc;
         h->cur ^= 1;
@@ -382,7 +403,7 @@ extern "C" int qcbf_rk3_step(qcbf_flow* h, double eps, int n_steps) {
 }
 
 static int ensure_force(qcbf_flow* h, int i) {
-    if (!h->F[i]) HIPCHK(h, hipMalloc(&h->F[i], (size_t)h->g.V * 16 * sizeof(double2)));
+    if (!h->F[i]) HIPCHK(h, hipMalloc(&h->F[i], qcb::alg_field_bytes(h->prec, h->g.V)));
     return QCBF_OK;
 }
 
@@ -391,15 +412,15 @@ extern "C" int qcbf_compute_force(qcbf_flow* h, int i) {
     if (i != 0 && i != 1) return fail(h, QCBF_ERR_ARG, "force index %d not 0 or 1", i);
     int rc = ensure_force(h, i);
     if (rc) return rc;
-    HIPCHK(h, qcb::launch_force_f64(h->U[h->cur], h->F[i], h->g, h->stream));
+    HIPCHK(h, qcb::launch_force(h->prec, h->U[h->cur], h->F[i], h->g, h->stream));
     return QCBF_OK;
 }
 
 static int update(qcbf_flow* h, int which, double eps) {
     if (!h->F[0] || (which > 1 && !h->F[1]))
         return fail(h, QCBF_ERR_STATE, "compute_w%d needs the forces it combines", which);
-    HIPCHK(h, qcb::launch_update_f64(which, h->U[h->cur], h->F[0], h->F[1], h->g, eps, h->dwarn,
-                                     h->stream));
+    HIPCHK(h, qcb::launch_update(h->prec, which, h->U[h->cur], h->F[0], h->F[1], h->g, eps,
+                                 h->dwarn, h->stream));
     return QCBF_OK;
 }
 extern "C" int qcbf_compute_w1(qcbf_flow* h, double eps) {
@@ -417,18 +438,22 @@ extern "C" int qcbf_compute_w3(qcbf_flow* h, double eps) {
 extern "C" int qcbf_replace_force_0(qcbf_flow* h) {
     CHECK_H(h);
     if (!h->F[0] || !h->F[1]) return fail(h, QCBF_ERR_STATE, "replace_force_0 needs both forces");
-    HIPCHK(h, qcb::launch_replace_force0_f64(h->F[0], h->F[1], h->g, h->stream));
+    HIPCHK(h, qcb::launch_replace_force0(h->prec, h->F[0], h->F[1], h->g, h->stream));
     return QCBF_OK;
 }
 extern "C" int qcbf_get_force(qcbf_flow* h, int i, double* out) {
     CHECK_H(h);
     if ((i != 0 && i != 1) || !out) return fail(h, QCBF_ERR_ARG, "bad force index or null output");
     if (!h->F[i]) return fail(h, QCBF_ERR_STATE, "force_%d has not been computed", i);
-    double* staging = reinterpret_cast<double*>(h->U[h->cur ^ 1]);
-    HIPCHK(h, qcb::launch_alg_to_aos_f64(h->F[i], staging, h->g, h->stream));
-    HIPCHK(h, hipMemcpyAsync(out, staging, (size_t)h->g.V * 72 * sizeof(double),
-                             hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    double* tmp = nullptr;
+    HIPCHK(h, hipMalloc(&tmp, (size_t)h->g.V * 72 * sizeof(double)));
+    hipError_t e = qcb::launch_alg_to_aos(h->prec, h->F[i], tmp, h->g, h->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, tmp, (size_t)h->g.V * 72 * sizeof(double), hipMemcpyDeviceToHost,
+                           h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(h, QCBF_ERR_HIP, "get_force: %s", hipGetErrorString(e));
     return QCBF_OK;
 }
 
@@ -437,7 +462,7 @@ static int measure_into(qcbf_flow* h, double* dout, bool clover) {
     qcbf_event* ev;
     int rc = timing_begin(h, QCBF_K_MEASURE, &ev);
     if (rc) return rc;
-    HIPCHK(h, qcb::launch_measure_f64(h->U[h->cur], h->g, h->partials, dout, clover, h->stream));
+    HIPCHK(h, qcb::launch_measure(h->prec, h->U[h->cur], h->g, h->partials, dout, clover, h->stream));
     return timing_end(h, ev);
 }
 
@@ -531,7 +556,7 @@ extern "C" int qcbf_exp_warnings(qcbf_flow* h, int* flagged) {
 extern "C" int qcbf_unitarity(qcbf_flow* h, double* max_dev, double* max_det_dev) {
     CHECK_H(h);
     if (!max_dev || !max_det_dev) return fail(h, QCBF_ERR_ARG, "null output");
-    HIPCHK(h, qcb::launch_unitarity_f64(h->U[h->cur], h->g, h->partials, h->dres, h->stream));
+    HIPCHK(h, qcb::launch_unitarity(h->prec, h->U[h->cur], h->g, h->partials, h->dres, h->stream));
     double r[2];
     HIPCHK(h, hipMemcpyAsync(r, h->dres, sizeof(r), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));

@@ -72,9 +72,12 @@ const char* qcbf_last_error(const qcbf_flow* h);
 
 /* GradientFlow(double epsilon, const GaugeField&, const GeometryCB&)
  * (src/flow/gradient_flow.cpp:12-13) + GeometryCB(L) (src/geometry/GeometryCB.cpp:5-9).
- * dims = {Lx, Ly, Lz, Lt}, each >= 2, product < 2^28 sites.  Allocates the link
- * field (cold start), its RK3 companion buffers and the measurement buffers on
- * the current HIP device. */
+ * dims = {Lx, Ly, Lz, Lt}, each >= 2, product < 2^28 sites.  precision is
+ * QCBF_FP64 or QCBF_FP32_MIXED.  Allocates the link field (cold start), its RK3
+ * companion buffers and the measurement buffers on the current HIP device.
+ * Every entry point below works in both precisions; host arrays are always
+ * FP64 (the FP32-mixed field is re-unitarised and rounded on upload and its
+ * third row rebuilt in FP64 on download). */
 int qcbf_create(const int dims[4], int precision, qcbf_flow** out);
 int qcbf_destroy(qcbf_flow* h);
 int qcbf_get_dims(const qcbf_flow* h, int dims[4], int* precision);
