@@ -22,123 +22,10 @@
 // uniform and the four wavefronts of one workgroup share neighbour links
 // through the CU's L1.  The workgroup -> site-tile order is XCD-aware
 // (tile_of_block, lattice.hpp).
-#include "fields.hpp"
+#include "flow_body.hpp"
 #include "kernels.hpp"
 
 namespace qcb {
-
-// Staple sum of U_mu(x):
-//   sum_{nu != mu} U_nu(x+mu) (U_nu(x) U_mu(x+nu))^dag
-//                + (U_mu(x-nu) U_nu(x+mu-nu))^dag U_nu(x-nu)
-// For FP64 one operand of every product is streamed from memory a row at a
-// time, so only the accumulator, one product and one full operand are live;
-// FP32 matrices are half the registers and are loaded whole.
-template <class F, int MU>
-__device__ inline mat3<typename F::real> staple_sum(const F& f, unsigned s, const nbr& n) {
-    using T = typename F::real;
-    mat3<T> st = zero3<T>();
-    const unsigned xmu = s + n.up[MU];
-#pragma unroll
-    for (int nu = 0; nu < 4; nu++) {
-        if (nu == MU) continue;
-        const unsigned xnu = s + n.up[nu];
-        const unsigned xmnu = s + n.dn[nu];
-        const unsigned xmunu = xmu + n.dn[nu];
-        if constexpr (F::stream_rows) {
-            {  // T = U_nu(x) U_mu(x+nu); st += U_nu(x+mu) T^dag
-                const mat3<T> B = f.link(MU, xnu);
-                mat3<T> P;
-#pragma unroll
-                for (int i = 0; i < 3; i++) {
-                    cpx<T> a[3];
-                    f.row(nu, s, i, a);
-#pragma unroll
-                    for (int j = 0; j < 3; j++)
-                        P.e[i][j] = madd(madd(a[0] * B.e[0][j], a[1], B.e[1][j]), a[2], B.e[2][j]);
-                }
-#pragma unroll
-                for (int i = 0; i < 3; i++) {
-                    cpx<T> c[3];
-                    f.row(nu, xmu, i, c);
-#pragma unroll
-                    for (int j = 0; j < 3; j++)
-                        st.e[i][j] = madd_c(madd_c(madd_c(st.e[i][j], c[0], P.e[j][0]), c[1],
-                                                   P.e[j][1]),
-                                            c[2], P.e[j][2]);
-                }
-            }
-            {  // T = U_mu(x-nu) U_nu(x+mu-nu); st += T^dag U_nu(x-nu)
-                const mat3<T> B = f.link(nu, xmunu);
-                mat3<T> P;
-#pragma unroll
-                for (int i = 0; i < 3; i++) {
-                    cpx<T> a[3];
-                    f.row(MU, xmnu, i, a);
-#pragma unroll
-                    for (int j = 0; j < 3; j++)
-                        P.e[i][j] = madd(madd(a[0] * B.e[0][j], a[1], B.e[1][j]), a[2], B.e[2][j]);
-                }
-#pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    cpx<T> v[3];
-                    f.row(nu, xmnu, k, v);  // row k of U_nu(x-nu)
-#pragma unroll
-                    for (int i = 0; i < 3; i++)
-#pragma unroll
-                        for (int j = 0; j < 3; j++) st.e[i][j] = madd_cl(st.e[i][j], P.e[k][i], v[j]);
-                }
-            }
-        } else {
-            {
-                const mat3<T> a = mul(f.link(nu, s), f.link(MU, xnu));
-                acc_mul_nd(st, f.link(nu, xmu), a);
-            }
-            {
-                const mat3<T> b = mul(f.link(MU, xmnu), f.link(nu, xmunu));
-                acc_mul_dn(st, b, f.link(nu, xmnu));
-            }
-        }
-    }
-    return st;
-}
-
-template <class F, int MU>
-__device__ inline alg8<typename F::real> force_at(const F& f, unsigned s, const nbr& n,
-                                                  mat3<typename F::real>& u) {
-    const mat3<typename F::real> st = staple_sum<F, MU>(f, s, n);
-    u = f.link(MU, s);
-    return neg_proj_ta(mul(u, st));
-}
-
-template <prec P, int STAGE, int MU>
-__device__ inline void stage_body(const typename storage<P>::field& f,
-                                  typename storage<P>::vec* __restrict__ Uout,
-                                  typename storage<P>::avec* __restrict__ acc, unsigned s,
-                                  const nbr& n, double cz, double ca, bool& warn) {
-    using S = storage<P>;
-    using T = typename S::real;
-    mat3<T> u;
-    const alg8<T> z = force_at<typename S::field, MU>(f, s, n, u);
-    const T czr = (T)cz, car = (T)ca;
-    alg8<T> x;
-    if constexpr (STAGE == 1) {
-#pragma unroll
-        for (int k = 0; k < 8; k++) x.a[k] = czr * z.a[k];
-        S::store_alg(acc, f.V, MU, s, z);
-    } else {
-        alg8<T> a = S::load_alg(acc, f.V, MU, s);
-#pragma unroll
-        for (int k = 0; k < 8; k++) x.a[k] = czr * z.a[k] - car * a.a[k];
-        if constexpr (STAGE == 2) {
-#pragma unroll
-            for (int k = 0; k < 8; k++)
-                a.a[k] = (T)(8.0 / 9.0) * z.a[k] - (T)(17.0 / 36.0) * a.a[k];
-            S::store_alg(acc, f.V, MU, s, a);
-        }
-    }
-    const mat3<T> e = exp_su3<T, double>(x, &warn);
-    S::store(Uout, f.V, MU, s, mul(e, u));
-}
 
 #ifndef QCB_STAGE_WAVES_PER_EU
 #define QCB_STAGE_WAVES_PER_EU 1

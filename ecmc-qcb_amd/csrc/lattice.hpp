@@ -49,7 +49,7 @@ __host__ __device__ inline unsigned tile_of_block(unsigned b, unsigned ntiles, i
     }
     return b;
 }
-inline unsigned grid_for_tiles(unsigned ntiles, int mode, unsigned win) {
+__host__ __device__ inline unsigned grid_for_tiles(unsigned ntiles, int mode, unsigned win) {
     if (mode == 1) return 8 * ((ntiles + 7) / 8);
     if (mode == 2) return ((ntiles + win - 1) / win) * win;
     return ntiles;

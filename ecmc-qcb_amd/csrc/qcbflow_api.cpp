@@ -705,3 +705,72 @@ extern "C" int qcbf_device_info(char* name, int name_len, int* compute_units) {
     if (compute_units) *compute_units = p.multiProcessorCount;
     return QCBF_OK;
 }
+
+// ---- autotuner ---------------------------------------------------------------------
+// Each candidate tile order is timed with HIP events over trial_steps RK3 steps
+// on the real field, which is saved first and restored afterwards.
+extern "C" int qcbf_autotune(qcbf_flow* h, double eps, int trial_steps, long* mode, long* win) {
+    CHECK_H(h);
+    if (trial_steps < 1) return fail(h, QCBF_ERR_ARG, "trial_steps < 1");
+    const size_t ub = qcb::link_field_bytes(h->prec, h->g.V);
+    const size_t ab = qcb::alg_field_bytes(h->prec, h->g.V);
+    void *saveU = nullptr, *saveA = nullptr;
+    HIPCHK(h, hipMalloc(&saveU, ub));
+    hipError_t e = hipMalloc(&saveA, ab);
+    if (e != hipSuccess) {
+        (void)hipFree(saveU);
+        return fail(h, QCBF_ERR_OOM, "autotune save buffer: %s", hipGetErrorString(e));
+    }
+    const int cur0 = h->cur;
+    const bool timing0 = h->timing;
+    h->timing = false;
+    const std::pair<int, unsigned> cands[] = {{0, 64},   {1, 64},   {2, 64},   {2, 128},
+                                              {2, 256},  {2, 512},  {2, 1024}, {2, 2048}};
+    std::pair<int, unsigned> best = {h->g.map_mode, h->g.map_win};
+    float best_ms = 1e30f;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = QCBF_OK;
+    if ((e = hipMemcpyAsync(saveU, h->U[cur0], ub, hipMemcpyDeviceToDevice, h->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(saveA, h->acc, ab, hipMemcpyDeviceToDevice, h->stream)) != hipSuccess ||
+        (e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess)
+        rc = fail(h, QCBF_ERR_HIP, "autotune setup: %s", hipGetErrorString(e));
+    for (const auto& c : cands) {
+        if (rc) break;
+        h->g.map_mode = c.first;
+        h->g.map_win = c.second;
+        rc = rk3_once(h, eps);  // warm
+        if (rc) break;
+        if ((e = hipEventRecord(e0, h->stream)) != hipSuccess) {
+            rc = fail(h, QCBF_ERR_HIP, "event: %s", hipGetErrorString(e));
+            break;
+        }
+        for (int s = 0; rc == QCBF_OK && s < trial_steps; s++) rc = rk3_once(h, eps);
+        if (rc) break;
+        float ms = 0.f;
+        if ((e = hipEventRecord(e1, h->stream)) != hipSuccess ||
+            (e = hipEventSynchronize(e1)) != hipSuccess ||
+            (e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) {
+            rc = fail(h, QCBF_ERR_HIP, "autotune timing: %s", hipGetErrorString(e));
+            break;
+        }
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = c;
+        }
+    }
+    h->g.map_mode = best.first;
+    h->g.map_win = best.second;
+    h->cur = cur0;
+    h->timing = timing0;
+    if ((e = hipMemcpyAsync(h->U[cur0], saveU, ub, hipMemcpyDeviceToDevice, h->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(h->acc, saveA, ab, hipMemcpyDeviceToDevice, h->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(h->stream)) != hipSuccess)
+        if (rc == QCBF_OK) rc = fail(h, QCBF_ERR_HIP, "autotune restore: %s", hipGetErrorString(e));
+    (void)hipFree(saveU);
+    (void)hipFree(saveA);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (mode) *mode = best.first;
+    if (win) *win = best.second;
+    return rc;
+}

@@ -31,7 +31,7 @@ FP64 = 0
 FP32_MIXED = 1
 
 K_STAGE1, K_STAGE2, K_STAGE3, K_MEASURE = 0, 1, 2, 3
-TUNE_MAP_MODE, TUNE_MAP_WIN = 0, 1
+TUNE_MAP_MODE, TUNE_MAP_WIN, TUNE_BLOCKS_PER_CU = 0, 1, 2
 K_COUNT = 4
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -69,6 +69,25 @@ SIGNATURES = {
     "qcbf_su3_exp": (ctypes.c_int, [_dp, ctypes.c_int, ctypes.c_double, _dp, _ip]),
     "qcbf_tune": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_long]),
     "qcbf_get_tune": (ctypes.c_int, [_vp, ctypes.c_int, _lp]),
+    "qcbf_autotune": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _lp, _lp]),
+    "qcbf_batch_last_error": (ctypes.c_char_p, [_vp]),
+    "qcbf_batch_create": (ctypes.c_int, [ctypes.c_int, _ip, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "qcbf_batch_destroy": (ctypes.c_int, [_vp]),
+    "qcbf_batch_size": (ctypes.c_int, [_vp, _ip, _ip]),
+    "qcbf_batch_set_links": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "qcbf_batch_get_links": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "qcbf_batch_hot_start": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_uint64]),
+    "qcbf_batch_rk3_step": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int]),
+    "qcbf_batch_measure": (ctypes.c_int, [_vp, _dp]),
+    "qcbf_batch_flow_measure": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                               _dp, _dp]),
+    "qcbf_batch_exp_warnings": (ctypes.c_int, [_vp, _ip]),
+    "qcbf_batch_synchronize": (ctypes.c_int, [_vp]),
+    "qcbf_batch_tune": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_long]),
+    "qcbf_batch_get_tune": (ctypes.c_int, [_vp, ctypes.c_int, _lp]),
+    "qcbf_batch_use_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "qcbf_batch_autotune": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _lp, _lp, _lp]),
+    "qcbf_batch_timing": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _lp]),
     "qcbf_timing_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "qcbf_timing_read": (ctypes.c_int, [_vp, _dp, _lp]),
     "qcbf_copy_bench": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_int, _dp]),
@@ -273,6 +292,13 @@ class GradientFlow:
         self._c(load().qcbf_get_tune(self._h, int(key), ctypes.byref(v)))
         return v.value
 
+    def autotune(self, trial_steps=3, eps=None):
+        """Pick the fastest tile order (field restored afterwards); returns (mode, win)."""
+        m, w = ctypes.c_long(), ctypes.c_long()
+        self._c(load().qcbf_autotune(self._h, self.epsilon if eps is None else float(eps),
+                                     int(trial_steps), ctypes.byref(m), ctypes.byref(w)))
+        return m.value, w.value
+
     # -- measurement support
     def timing(self, on=True):
         self._c(load().qcbf_timing_enable(self._h, 1 if on else 0))
@@ -287,6 +313,109 @@ class GradientFlow:
         ms = ctypes.c_double()
         self._c(load().qcbf_copy_bench(self._h, int(nbytes), int(reps), ctypes.byref(ms)))
         return ms.value
+
+
+class Ensemble:
+    """Configurations of uneven volumes flowed together (one persistent kernel
+    per RK3 stage over all of them; optional hipGraph replay of each step)."""
+
+    def __init__(self, dims, epsilon=0.02, precision=FP64):
+        self.dims = [_dims(d) for d in dims]
+        self.n = len(self.dims)
+        self.V = [int(np.prod(d)) for d in self.dims]
+        self.epsilon = float(epsilon)
+        flat = (ctypes.c_int * (4 * self.n))(*[v for d in self.dims for v in d])
+        h = _vp()
+        rc = load().qcbf_batch_create(self.n, flat, int(precision), ctypes.byref(h))
+        if rc != QCBF_OK:
+            raise QcbfError(rc, load().qcbf_batch_last_error(None).decode())
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().qcbf_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _c(self, rc):
+        if rc != QCBF_OK:
+            raise QcbfError(rc, load().qcbf_batch_last_error(self._h).decode())
+
+    def copy(self, c, links):
+        links = np.ascontiguousarray(links, dtype=np.float64)
+        if links.size != self.V[c] * 72:
+            raise ValueError("wrong number of doubles for configuration %d" % c)
+        self._c(load().qcbf_batch_set_links(self._h, int(c), _p(links)))
+
+    def field(self, c):
+        out = np.empty((self.V[c], 4, 3, 3, 2))
+        self._c(load().qcbf_batch_get_links(self._h, int(c), _p(out)))
+        return out
+
+    def hot_start(self, c, seed):
+        self._c(load().qcbf_batch_hot_start(self._h, int(c), int(seed)))
+
+    def rk3_step(self, n=1, eps=None):
+        self._c(load().qcbf_batch_rk3_step(self._h, self.epsilon if eps is None else float(eps),
+                                           int(n)))
+
+    def measure(self):
+        out = np.empty(3 * self.n)
+        self._c(load().qcbf_batch_measure(self._h, _p(out)))
+        return out.reshape(self.n, 3)
+
+    def flow_measure(self, n_steps_gf, n_rk_steps, eps=None):
+        """Returns (tQE of shape (n_cfg, n_steps_gf, 3), plaquettes (n_cfg, n_steps_gf))."""
+        tqe = np.empty(3 * n_steps_gf * self.n)
+        plaq = np.empty(n_steps_gf * self.n)
+        self._c(load().qcbf_batch_flow_measure(self._h, self.epsilon if eps is None else float(eps),
+                                               int(n_steps_gf), int(n_rk_steps), _p(tqe),
+                                               _p(plaq)))
+        return tqe.reshape(self.n, n_steps_gf, 3), plaq.reshape(self.n, n_steps_gf)
+
+    def exp_warnings(self):
+        f = ctypes.c_int()
+        self._c(load().qcbf_batch_exp_warnings(self._h, ctypes.byref(f)))
+        return f.value
+
+    def synchronize(self):
+        self._c(load().qcbf_batch_synchronize(self._h))
+
+    def tune(self, key, value):
+        self._c(load().qcbf_batch_tune(self._h, int(key), int(value)))
+
+    def get_tune(self, key):
+        v = ctypes.c_long()
+        self._c(load().qcbf_batch_get_tune(self._h, int(key), ctypes.byref(v)))
+        return v.value
+
+    def use_graph(self, on=True):
+        self._c(load().qcbf_batch_use_graph(self._h, 1 if on else 0))
+
+    def autotune(self, trial_steps=2, eps=None):
+        m, w, b = ctypes.c_long(), ctypes.c_long(), ctypes.c_long()
+        self._c(load().qcbf_batch_autotune(self._h, self.epsilon if eps is None else float(eps),
+                                           int(trial_steps), ctypes.byref(m), ctypes.byref(w),
+                                           ctypes.byref(b)))
+        return m.value, w.value, b.value
+
+    def timing(self, on=True):
+        """Returns (ms, rk3 steps) of flow_measure calls since the previous call."""
+        ms, n = ctypes.c_double(), ctypes.c_long()
+        self._c(load().qcbf_batch_timing(self._h, 1 if on else 0, ctypes.byref(ms),
+                                         ctypes.byref(n)))
+        return ms.value, n.value
 
 
 # ---- mpi::observables mirrors (single device; the topology argument of the

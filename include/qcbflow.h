@@ -161,13 +161,54 @@ enum qcbf_tune_key {
     QCBF_TUNE_MAP_MODE = 0, /* workgroup -> site-tile order of the flow kernels:
                                0 linear, 1 one contiguous chunk per XCD,
                                2 XCD chunks inside windows of MAP_WIN workgroups */
-    QCBF_TUNE_MAP_WIN = 1   /* window for mode 2: a multiple of 8, >= 8 */
+    QCBF_TUNE_MAP_WIN = 1,  /* window for mode 2: a multiple of 8, >= 8 */
+    QCBF_TUNE_BLOCKS_PER_CU = 2 /* ensembles only: persistent-grid workgroups per CU */
 };
 /* Set one launch parameter of h; results do not depend on it (only the order in
  * which workgroups visit sites changes).  QCBF_ERR_ARG for an unknown key or
  * value. */
 int qcbf_tune(qcbf_flow* h, int key, long value);
 int qcbf_get_tune(const qcbf_flow* h, int key, long* value);
+
+/* Time each candidate launch shape of the flow kernels for trial_steps RK3
+ * steps on a copy of the current field (the field is restored afterwards) and
+ * keep the fastest; *mode and *win (may be NULL) receive the choice. */
+int qcbf_autotune(qcbf_flow* h, double eps, int trial_steps, long* mode, long* win);
+
+/* ---- ensembles (no counterpart in the reference: its callers flow one
+ * configuration at a time through topo_charge_flowed) ---------------------- */
+/* n configurations of extents dims[4c..4c+3] (each >= 2, volume < 2^28), one
+ * precision.  All fields live in one allocation; one persistent kernel per RK3
+ * stage walks the (configuration, 64-site tile) list of all of them. */
+typedef struct qcbf_batch qcbf_batch;
+const char* qcbf_batch_last_error(const qcbf_batch* b);
+int qcbf_batch_create(int n, const int* dims, int precision, qcbf_batch** out);
+int qcbf_batch_destroy(qcbf_batch* b);
+int qcbf_batch_size(const qcbf_batch* b, int* n, int* dims);
+int qcbf_batch_set_links(qcbf_batch* b, int c, const double* links);
+int qcbf_batch_get_links(qcbf_batch* b, int c, double* links);
+int qcbf_batch_hot_start(qcbf_batch* b, int c, uint64_t seed);
+int qcbf_batch_rk3_step(qcbf_batch* b, double eps, int n_steps);
+/* {P, E, Q} of every configuration: out[3c..3c+2] */
+int qcbf_batch_measure(qcbf_batch* b, double* out);
+/* topo_charge_flowed cadence for every configuration at once:
+ * tQE[(c*n_steps_gf + n)*3 + {0,1,2}] = (t, Q, E), plaq[c*n_steps_gf + n]
+ * (plaq may be NULL).  With graphs on and n_rk_steps = 1, each (measure, RK3
+ * step) pair is one replay of a captured hipGraph. */
+int qcbf_batch_flow_measure(qcbf_batch* b, double eps, int n_steps_gf, int n_rk_steps,
+                            double* tQE, double* plaq);
+int qcbf_batch_exp_warnings(qcbf_batch* b, int* flagged);
+int qcbf_batch_synchronize(qcbf_batch* b);
+int qcbf_batch_tune(qcbf_batch* b, int key, long value);
+int qcbf_batch_get_tune(const qcbf_batch* b, int key, long* value);
+int qcbf_batch_use_graph(qcbf_batch* b, int on);
+/* Autotuner: tile order x persistent-grid size, timed on a copy of the fields
+ * (restored afterwards); the fastest is kept. */
+int qcbf_batch_autotune(qcbf_batch* b, double eps, int trial_steps, long* mode, long* win,
+                        long* blocks_per_cu);
+/* Returns the device time of the flow_measure calls since the previous call
+ * (total ms and number of RK3 steps), then switches the timer on or off. */
+int qcbf_batch_timing(qcbf_batch* b, int on, double* total_ms, long* rk3_steps);
 
 /* ---- measurement support (no counterpart in the reference) -------------- */
 /* When on, every fused-stage and measurement launch on h is bracketed by HIP

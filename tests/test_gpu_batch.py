@@ -1,0 +1,126 @@
+"""Ensembles of uneven volumes flowed together (persistent stage kernels,
+hipGraph-replayed steps, autotuned launch shape) against the single-lattice
+path and the oracle."""
+import numpy as np
+import pytest
+
+import oracle
+import qcbflow
+
+pytestmark = pytest.mark.gpu
+
+DIMS = [(4, 4, 4, 4), (6, 4, 5, 8), (8, 8, 8, 8), (3, 5, 4, 7), (4, 4, 4, 4)]
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def fields_for(dims, sweeps=3):
+    out = []
+    for k, L in enumerate(dims):
+        f = oracle.hot_start(L, 500 + k)
+        out.append(oracle.heatbath(f, L, 6.0, sweeps, 900 + k))
+    return out
+
+
+@pytest.mark.parametrize("prec", [qcbflow.FP64, qcbflow.FP32_MIXED])
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_equals_single_lattice_path(prec, graph):
+    """Each member of the ensemble gets bit-for-bit the series the
+    single-lattice path gives (same per-link code, same reduction order)."""
+    say(f"ensemble of {len(DIMS)} uneven lattices, precision {prec}, graph {graph}")
+    fs = fields_for(DIMS)
+    with qcbflow.Ensemble(DIMS, 0.02, prec) as ens:
+        ens.use_graph(graph)
+        for c, f in enumerate(fs):
+            ens.copy(c, f)
+        tqe, plaq = ens.flow_measure(9, 1)
+        outs = [ens.field(c) for c in range(len(DIMS))]
+    for c, (L, f) in enumerate(zip(DIMS, fs)):
+        with qcbflow.GradientFlow(L, 0.02, prec) as gf:
+            gf.copy(f)
+            t1, p1 = gf.flow_measure(9, 1)
+            assert np.array_equal(tqe[c], t1), c
+            assert np.array_equal(plaq[c], p1), c
+            assert np.array_equal(outs[c], gf.field()), c
+
+
+def test_batch_fp64_against_oracle():
+    fs = fields_for(DIMS)
+    with qcbflow.Ensemble(DIMS, 0.02) as ens:
+        ens.use_graph(True)
+        for c, f in enumerate(fs):
+            ens.copy(c, f)
+        tqe, plaq = ens.flow_measure(11, 2)
+    for c, (L, f) in enumerate(zip(DIMS, fs)):
+        ref, refp, _ = oracle.flow_measure(f, L, 0.02, 11, 2)
+        assert np.array_equal(tqe[c][:, 0], ref[:, 0])
+        assert np.max(np.abs(tqe[c][:, 2] - ref[:, 2]) / ref[:, 2]) < 1e-12
+        assert np.max(np.abs(tqe[c][:, 1] - ref[:, 1])) < 1e-10
+        assert np.max(np.abs(plaq[c] - refp) / refp) < 1e-12
+
+
+def test_graph_replays_match_eager_over_many_steps():
+    """Graph replays interleaved with eager work and synchronisations give the
+    eager results at every step."""
+    dims = [(8, 8, 8, 8), (12, 12, 12, 12), (4, 6, 8, 10)]
+    with qcbflow.Ensemble(dims) as a, qcbflow.Ensemble(dims) as b:
+        for c in range(len(dims)):
+            a.hot_start(c, 40 + c)
+            b.hot_start(c, 40 + c)
+        b.use_graph(True)
+        for rep in range(3):
+            ta, pa = a.flow_measure(6, 1)
+            tb, pb = b.flow_measure(6, 1)
+            assert np.array_equal(ta, tb) and np.array_equal(pa, pb), rep
+            a.measure()
+            b.measure()
+            b.synchronize()
+
+
+def test_autotune_leaves_the_ensemble_unchanged():
+    dims = [(8, 8, 8, 8), (12, 12, 12, 12)]
+    with qcbflow.Ensemble(dims) as ens:
+        for c in range(len(dims)):
+            ens.hot_start(c, 7 + c)
+        before = [ens.field(c) for c in range(len(dims))]
+        m, w, bpc = ens.autotune(trial_steps=1)
+        say(f"  autotune picked mode {m} win {w} blocks/CU {bpc}")
+        assert ens.get_tune(qcbflow.TUNE_MAP_MODE) == m
+        for c in range(len(dims)):
+            assert np.array_equal(ens.field(c), before[c])
+        t1, _ = ens.flow_measure(3, 1)
+    with qcbflow.Ensemble(dims) as ref:
+        for c in range(len(dims)):
+            ref.hot_start(c, 7 + c)
+        t0, _ = ref.flow_measure(3, 1)
+    assert np.array_equal(t0, t1)
+
+
+def test_single_lattice_autotune_and_tile_maps_agree():
+    L = (16, 8, 8, 12)
+    with qcbflow.GradientFlow(L) as gf:
+        gf.hot_start(3)
+        f = gf.field()
+        m, w = gf.autotune(trial_steps=1)
+        assert np.array_equal(gf.field(), f)
+        results = []
+        for mode, win in [(0, 64), (1, 64), (2, 64), (2, 1024)]:
+            gf.tune(qcbflow.TUNE_MAP_MODE, mode)
+            gf.tune(qcbflow.TUNE_MAP_WIN, win)
+            gf.copy(f)
+            results.append(gf.flow_measure(4, 1))
+    for r in results[1:]:
+        assert np.array_equal(r[0], results[0][0]) and np.array_equal(r[1], results[0][1])
+
+
+def test_batch_errors():
+    with pytest.raises(qcbflow.QcbfError) as ei:
+        qcbflow.Ensemble([(4, 4, 4, 4), (4, 1, 4, 4)])
+    assert ei.value.code == qcbflow.QCBF_ERR_ARG
+    with qcbflow.Ensemble([(4, 4, 4, 4)]) as ens:
+        with pytest.raises(qcbflow.QcbfError):
+            ens.hot_start(3, 1)
+        with pytest.raises(qcbflow.QcbfError):
+            ens.tune(qcbflow.TUNE_BLOCKS_PER_CU, 0)
