@@ -62,6 +62,12 @@ hipError_t launch_alg_to_aos(prec p, const void* alg, double* aos, const lattice
 hipError_t launch_hot_start(prec p, void* U, const lattice& g, uint64_t seed, hipStream_t st);
 hipError_t launch_cold_start(prec p, void* U, const lattice& g, hipStream_t st);
 
+// n_sweeps checkerboard Cabibbo-Marinari heatbath sweeps at beta on an FP64
+// field with even extents; sweep numbers sweep0.. key the random streams.
+// *stuck is set if a Kennedy-Pendleton draw hit its retry bound.
+hipError_t launch_heatbath_f64(double2* U, const lattice& g, double beta, int n_sweeps,
+                               uint64_t seed, unsigned sweep0, int* stuck, hipStream_t st);
+
 // exp test entry: n matrices Z (18 doubles each, AoS) -> exp(coeff * P(Z)) where
 // P keeps the traceless anti-hermitian entries Z is assumed to have.
 hipError_t launch_exp_batch_f64(const double* Z, double* out, int n, double coeff, int* warn,

@@ -40,6 +40,7 @@ struct qcbf_flow {
     double* dres = nullptr;           // device results of measurements
     size_t dres_cap = 0;              // in doubles
     int* dwarn = nullptr;             // exp t > 1 flag
+    unsigned hb_sweeps = 0;           // heatbath sweeps done (keys the random streams)
     void* stage = nullptr;            // device staging chunk for host <-> HBM transfers
     std::string err;
     // timing
@@ -255,6 +256,27 @@ extern "C" int qcbf_get_links(qcbf_flow* h, double* links) {
 extern "C" int qcbf_hot_start(qcbf_flow* h, uint64_t seed) {
     CHECK_H(h);
     HIPCHK(h, qcb::launch_hot_start(h->prec, h->U[h->cur], h->g, seed, h->stream));
+    return QCBF_OK;
+}
+
+extern "C" int qcbf_heatbath(qcbf_flow* h, double beta, int n_sweeps, uint64_t seed) {
+    CHECK_H(h);
+    if (h->prec != qcb::prec::f64)
+        return fail(h, QCBF_ERR_STATE, "the heatbath runs on FP64 fields only");
+    for (int d = 0; d < 4; d++)
+        if (h->g.L[d] % 2)
+            return fail(h, QCBF_ERR_ARG, "the checkerboard heatbath needs even extents (L[%d] = %d)",
+                        d, h->g.L[d]);
+    if (n_sweeps < 0 || !(beta > 0.0)) return fail(h, QCBF_ERR_ARG, "bad heatbath arguments");
+    HIPCHK(h, hipMemsetAsync(h->dwarn, 0, sizeof(int), h->stream));
+    HIPCHK(h, qcb::launch_heatbath_f64(static_cast<double2*>(h->U[h->cur]), h->g, beta, n_sweeps,
+                                       seed, h->hb_sweeps, h->dwarn, h->stream));
+    h->hb_sweeps += (unsigned)n_sweeps;
+    int stuck = 0;
+    HIPCHK(h, hipMemcpyAsync(&stuck, h->dwarn, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemsetAsync(h->dwarn, 0, sizeof(int), h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (stuck) return fail(h, QCBF_ERR_STATE, "a Kennedy-Pendleton draw hit its retry bound");
     return QCBF_OK;
 }
 

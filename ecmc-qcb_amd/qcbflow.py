@@ -49,6 +49,7 @@ SIGNATURES = {
     "qcbf_get_links": (ctypes.c_int, [_vp, _dp]),
     "qcbf_hot_start": (ctypes.c_int, [_vp, ctypes.c_uint64]),
     "qcbf_cold_start": (ctypes.c_int, [_vp]),
+    "qcbf_heatbath": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_uint64]),
     "qcbf_load_ildg": (ctypes.c_int, [_vp, ctypes.c_char_p]),
     "qcbf_save_ildg": (ctypes.c_int, [_vp, ctypes.c_char_p]),
     "qcbf_rk3_step": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int]),
@@ -214,6 +215,9 @@ class GradientFlow:
 
     def cold_start(self):
         self._c(load().qcbf_cold_start(self._h))
+
+    def heatbath(self, beta, n_sweeps, seed):
+        self._c(load().qcbf_heatbath(self._h, float(beta), int(n_sweeps), int(seed)))
 
     def load_ildg(self, path):
         self._c(load().qcbf_load_ildg(self._h, os.fsencode(path)))

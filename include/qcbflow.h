@@ -94,6 +94,15 @@ int qcbf_get_links(qcbf_flow* h, double* links);
 int qcbf_hot_start(qcbf_flow* h, uint64_t seed);
 int qcbf_cold_start(qcbf_flow* h);
 
+/* n_sweeps Cabibbo-Marinari heatbath sweeps at beta on the flowing field
+ * (src/heatbath/heatbath.cpp:8-44, src/heatbath/heatbath_mpi.cpp:10-33): the
+ * step that thermalises the flow's inputs, here on the device.  Checkerboard
+ * by (mu, site parity), so the extents must be even; FP64 handles only
+ * (QCBF_ERR_STATE otherwise).  Links are re-unitarised after every sweep.
+ * Successive calls continue one random stream keyed by seed and the running
+ * sweep count. */
+int qcbf_heatbath(qcbf_flow* h, double beta, int n_sweeps, uint64_t seed);
+
 /* read_ildg_clime (src/io/ildg.cpp:138-237): load an ILDG/LIME file straight
  * into the flowing field.  path is the file itself (the reference's
  * dirpath/filename/filename composition lives in the C++ drop-in).  The

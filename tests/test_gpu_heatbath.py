@@ -1,0 +1,67 @@
+"""Device heatbath (the thermalisation step before the flow): statistics at
+known couplings, agreement with the oracle's heatbath, reproducibility,
+unitarity, and its argument checks."""
+import numpy as np
+import pytest
+
+import oracle
+import qcbflow
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("beta,lo,hi", [(5.7, 0.535, 0.565), (6.0, 0.585, 0.605),
+                                         (6.2, 0.605, 0.625)])
+def test_plaquette_at_known_couplings(beta, lo, hi):
+    """<P> ~ 0.549 (5.7), 0.594 (6.0), 0.614 (6.2) on large lattices."""
+    L = (8, 8, 8, 8)
+    with qcbflow.GradientFlow(L) as gf:
+        gf.hot_start(1)
+        gf.heatbath(beta, 30, 77)
+        ps = []
+        for _ in range(10):
+            gf.heatbath(beta, 2, 77)
+            ps.append(gf.plaquette())
+        du, dd = gf.unitarity()
+    P = float(np.mean(ps))
+    say(f"  beta {beta}: <P> = {P:.5f}")
+    assert lo < P < hi and du < 1e-13 and dd < 1e-13
+
+
+def test_agrees_with_oracle_heatbath():
+    L = (6, 6, 6, 6)
+    f = oracle.thermalised(L, 6.0, 1234, 40)
+    Pc = oracle.plaquette(f, L)
+    with qcbflow.GradientFlow(L) as gf:
+        gf.copy(oracle.hot_start(L, 1234))
+        gf.heatbath(6.0, 40, 99)
+        Pg = gf.plaquette()
+    say(f"  oracle P = {Pc:.5f}, device P = {Pg:.5f}")
+    assert abs(Pc - Pg) < 0.012
+
+
+def test_reproducible_and_seeded():
+    L = (8, 4, 6, 4)
+    outs = []
+    for seed in (5, 5, 6):
+        with qcbflow.GradientFlow(L) as gf:
+            gf.hot_start(2)
+            gf.heatbath(6.0, 3, seed)
+            outs.append(gf.field())
+    assert np.array_equal(outs[0], outs[1])
+    assert not np.array_equal(outs[0], outs[2])
+
+
+def test_heatbath_argument_checks():
+    with qcbflow.GradientFlow((4, 4, 5, 4)) as gf:
+        with pytest.raises(qcbflow.QcbfError) as ei:
+            gf.heatbath(6.0, 1, 1)
+        assert ei.value.code == qcbflow.QCBF_ERR_ARG
+    with qcbflow.GradientFlow((4, 4, 4, 4), precision=qcbflow.FP32_MIXED) as gf:
+        with pytest.raises(qcbflow.QcbfError) as ei:
+            gf.heatbath(6.0, 1, 1)
+        assert ei.value.code == qcbflow.QCBF_ERR_STATE
