@@ -110,8 +110,12 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-mixed leg")
-    ap.add_argument("--map-mode", type=int, default=0, help="workgroup->tile order (qcbf_tune)")
-    ap.add_argument("--map-win", type=int, default=64)
+    ap.add_argument("--map-mode", type=int, default=None,
+                    help="workgroup->tile order (qcbf_tune); default: the library's")
+    ap.add_argument("--map-win", type=int, default=None)
+    ap.add_argument("--tile-shape", type=int, default=None, help="qcbf_tune tile-shape code")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip qcbf_autotune before the warmup")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run the plain copy kernel on this many MiB (traffic calibration)")
     args = ap.parse_args()
@@ -129,10 +133,22 @@ def main():
 
     name, cus = qcbflow.device_info()
     log(f"device {name}, {cus} CUs; lattice {L}, V = {V}")
+    def setup(h):
+        """launch shape: explicit knobs, then (unless disabled) the autotuner"""
+        for key, val in ((qcbflow.TUNE_MAP_MODE, args.map_mode),
+                         (qcbflow.TUNE_MAP_WIN, args.map_win),
+                         (qcbflow.TUNE_TILE_SHAPE, args.tile_shape)):
+            if val is not None:
+                h.tune(key, val)
+        if not args.no_autotune:
+            h.autotune(trial_steps=2)
+        return [h.get_tune(k) for k in (qcbflow.TUNE_MAP_MODE, qcbflow.TUNE_MAP_WIN,
+                                        qcbflow.TUNE_TILE_SHAPE)]
+
     gf = qcbflow.GradientFlow(L, args.eps)
     gf.hot_start(args.seed)
-    for key, val in ((qcbflow.TUNE_MAP_MODE, args.map_mode), (qcbflow.TUNE_MAP_WIN, args.map_win)):
-        gf.tune(key, val)
+    shape64 = setup(gf)
+    log(f"fp64 launch shape (map mode, window, tile code): {shape64}")
 
     # timed region: K RK3 steps, K+1 measurements, stage kernels bracketed by events
     K = args.steps
@@ -170,8 +186,7 @@ def main():
         links = gf.field() if not args.no_cpu else None
         gm = qcbflow.GradientFlow(L, args.eps, qcbflow.FP32_MIXED)
         gm.hot_start(args.seed)
-        gm.tune(qcbflow.TUNE_MAP_MODE, args.map_mode)
-        gm.tune(qcbflow.TUNE_MAP_WIN, args.map_win)
+        shape32 = setup(gm)
         msecs, mtqe, mplaq, mms, mn = timed_flow(gm, K, args.warmup)
         mstage = (mms[0] + mms[1] + mms[2]) / (mn[0] + mn[1] + mn[2])
         mach = V * MIXED_BYTES_PER_SITE_STEP / 3.0 / (mstage * 1e-3) / 1e9
@@ -187,6 +202,7 @@ def main():
             "final_t": float(mtqe[-1, 0]),
             "dE_rel_vs_fp64_at_final_t": float(abs(mtqe[-1, 2] - tqe[-1, 2]) / tqe[-1, 2]),
             "dQ_vs_fp64_at_final_t": float(abs(mtqe[-1, 1] - tqe[-1, 1])),
+            "launch_shape": shape32,
             "note": "same Haar hot start rounded to FP32; accuracy gates on thermalised "
                     "lattices are in tests/test_gpu_mixed.py",
         }
@@ -229,7 +245,7 @@ def main():
             "measure_every_rk3_steps": 1,
             "precision": "fp64",
             "parallelism": "single GPU",
-            "tile_map": [args.map_mode, args.map_win],
+            "launch_shape": shape64,
         },
         "roofline": {
             "bound": "hbm",

@@ -29,9 +29,10 @@ __global__ __launch_bounds__(256) void k_batch_stage(const typename storage<P>::
         const unsigned tile = tile_of_block(v, ntiles, map_mode, map_win);
         if (tile >= ntiles) continue;
         const cfg_desc& c = cfgs[tile_cfg[tile]];
-        const unsigned s = (tile - c.tile0) * kSitesPerBlock + threadIdx.x;
+        unsigned co[4];
+        const unsigned s = tile_site(c.g, tile - c.tile0, threadIdx.x, co);
         if (s >= c.g.V) continue;
-        const nbr n = neighbours(c.g, s);
+        const nbr n = neighbours_at(c.g, co);
         const auto f = storage<P>::view(Uin + c.u_off, c.g.V);
         auto* out = Uout + c.u_off;
         auto* a = acc + c.a_off;

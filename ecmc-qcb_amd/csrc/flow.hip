@@ -36,11 +36,13 @@ __global__ __launch_bounds__(256, QCB_STAGE_WAVES_PER_EU) void k_rk3_stage(
     const typename storage<P>::vec* __restrict__ Uin, typename storage<P>::vec* __restrict__ Uout,
     typename storage<P>::avec* __restrict__ acc, lattice g, double cz, double ca,
     int* __restrict__ warn) {
-    const unsigned ntiles = (g.V + kSitesPerBlock - 1) / kSitesPerBlock;
+    const unsigned ntiles = stage_tiles(g);
     const unsigned tile = tile_of_block(blockIdx.x, ntiles, g.map_mode, g.map_win);
-    const unsigned s = tile * kSitesPerBlock + threadIdx.x;
-    if (tile >= ntiles || s >= g.V) return;
-    const nbr n = neighbours(g, s);
+    if (tile >= ntiles) return;
+    unsigned c[4];
+    const unsigned s = tile_site(g, tile, threadIdx.x, c);
+    if (s >= g.V) return;
+    const nbr n = neighbours_at(g, c);
     const auto f = storage<P>::view(Uin, g.V);
     bool w = false;
     switch (threadIdx.y) {
@@ -60,8 +62,7 @@ static hipError_t stage_launch(int stage, const void* Uin, void* Uout, void* acc
     auto* out = static_cast<typename S::vec*>(Uout);
     auto* a = static_cast<typename S::avec*>(acc);
     const dim3 block(kSitesPerBlock, 4);
-    const dim3 grid(grid_for_tiles((g.V + kSitesPerBlock - 1) / kSitesPerBlock, g.map_mode,
-                                   g.map_win));
+    const dim3 grid(grid_for_tiles(stage_tiles(g), g.map_mode, g.map_win));
     switch (stage) {
         case 1: k_rk3_stage<P, 1><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
         case 2: k_rk3_stage<P, 2><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;

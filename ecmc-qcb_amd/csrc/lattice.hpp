@@ -27,7 +27,38 @@ struct lattice {
     // workgroup -> site-tile order of the flow kernels (see tile_of_block)
     int map_mode;       // 0 linear, 1 one contiguous chunk per XCD, 2 XCD chunks per window
     unsigned map_win;   // window in workgroups for map_mode 2 (multiple of 8)
+    // Shape of the 64-site tile of one stage workgroup: lg_tile[d] = log2 of its
+    // extent along d (sum 6), or lg_tile[0] < 0 for 64 consecutive sites.  A
+    // box tile needs every L[d] divisible by its extent.
+    int lg_tile[4];
+    unsigned ntile[4];  // tiles along each direction (box tiles)
 };
+
+// Site and coordinates of lane `lane` (0..63) of flow-kernel tile `tile`.
+__host__ __device__ inline unsigned tile_site(const lattice& g, unsigned tile, unsigned lane,
+                                     unsigned (&c)[4]) {
+    if (g.lg_tile[0] < 0) {
+        const unsigned s = tile * 64u + lane;
+        unsigned r = s;
+        c[0] = r % (unsigned)g.L[0];
+        r /= (unsigned)g.L[0];
+        c[1] = r % (unsigned)g.L[1];
+        r /= (unsigned)g.L[1];
+        c[2] = r % (unsigned)g.L[2];
+        c[3] = r / (unsigned)g.L[2];
+        return s;
+    }
+    unsigned r = tile, l = lane;
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+        const unsigned td = (d < 3) ? r % g.ntile[d] : r;
+        r = (d < 3) ? r / g.ntile[d] : 0u;
+        const unsigned ld = l & ((1u << g.lg_tile[d]) - 1u);
+        l >>= g.lg_tile[d];
+        c[d] = (td << g.lg_tile[d]) + ld;
+    }
+    return ((c[3] * (unsigned)g.L[2] + c[2]) * (unsigned)g.L[1] + c[1]) * (unsigned)g.L[0] + c[0];
+}
 
 // The dispatcher hands workgroups to the 8 XCDs round-robin (workgroup b runs
 // on XCD b % 8), and each XCD has its own L2.  These orders give every XCD a
@@ -64,8 +95,13 @@ inline lattice make_lattice(const int dims[4]) {
         st *= (unsigned)dims[d];
     }
     g.V = st;
-    g.map_mode = 0;
-    g.map_win = 64;
+    // measured best on MI355X at 32^4 and 48^4 (qcbf_autotune refines it)
+    g.map_mode = 2;
+    g.map_win = 1024;
+    for (int d = 0; d < 4; d++) {
+        g.lg_tile[d] = d == 0 ? -1 : 0;
+        g.ntile[d] = 0;
+    }
     return g;
 }
 
@@ -74,6 +110,18 @@ struct nbr {
     int up[4];
     int dn[4];
 };
+
+__device__ inline nbr neighbours_at(const lattice& g, const unsigned (&c)[4]) {
+    nbr n;
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+        const int st = (int)g.stride[d];
+        const int wrap = (g.L[d] - 1) * st;
+        n.up[d] = (c[d] == (unsigned)(g.L[d] - 1)) ? -wrap : st;
+        n.dn[d] = (c[d] == 0u) ? wrap : -st;
+    }
+    return n;
+}
 
 __device__ inline nbr neighbours(const lattice& g, unsigned s) {
     unsigned c[4];
@@ -84,15 +132,30 @@ __device__ inline nbr neighbours(const lattice& g, unsigned s) {
     r /= (unsigned)g.L[1];
     c[2] = r % (unsigned)g.L[2];
     c[3] = r / (unsigned)g.L[2];
-    nbr n;
-#pragma unroll
-    for (int d = 0; d < 4; d++) {
-        const int st = (int)g.stride[d];
-        const int wrap = (g.L[d] - 1) * st;
-        n.up[d] = (c[d] == (unsigned)(g.L[d] - 1)) ? -wrap : st;
-        n.dn[d] = (c[d] == 0u) ? wrap : -st;
+    return neighbours_at(g, c);
+}
+
+// Set the stage-tile shape (log2 extents summing to 6) if every extent is
+// divisible by it; lg = nullptr (or a shape that does not divide the lattice)
+// selects 64 consecutive sites.  Returns whether the box shape was taken.
+inline bool set_tile_shape(lattice& g, const int* lg) {
+    bool ok = lg != nullptr;
+    if (ok) {
+        int sum = 0;
+        for (int d = 0; d < 4; d++) {
+            ok = ok && lg[d] >= 0 && lg[d] <= 6 && g.L[d] % (1 << lg[d]) == 0;
+            sum += lg[d];
+        }
+        ok = ok && sum == 6;
     }
-    return n;
+    for (int d = 0; d < 4; d++) {
+        g.lg_tile[d] = ok ? lg[d] : (d == 0 ? -1 : 0);
+        g.ntile[d] = ok ? (unsigned)(g.L[d] >> lg[d]) : 0u;
+    }
+    return ok;
+}
+__host__ __device__ inline unsigned stage_tiles(const lattice& g) {
+    return g.lg_tile[0] < 0 ? (g.V + 63) / 64 : g.ntile[0] * g.ntile[1] * g.ntile[2] * g.ntile[3];
 }
 
 // ---- FP64 structure-of-arrays link field -----------------------------------

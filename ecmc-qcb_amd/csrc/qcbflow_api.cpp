@@ -637,9 +637,27 @@ extern "C" int qcbf_tune(qcbf_flow* h, int key, long value) {
                 return fail(h, QCBF_ERR_ARG, "map window %ld not a multiple of 8 in [8, 2^24]", value);
             h->g.map_win = (unsigned)value;
             return QCBF_OK;
+        case QCBF_TUNE_TILE_SHAPE: {
+            if (value == -1) {
+                qcb::set_tile_shape(h->g, nullptr);
+                return QCBF_OK;
+            }
+            const int lg[4] = {(int)(value & 7), (int)((value >> 3) & 7), (int)((value >> 6) & 7),
+                               (int)((value >> 9) & 7)};
+            if (value < 0 || value >= 4096 || !qcb::set_tile_shape(h->g, lg)) {
+                qcb::set_tile_shape(h->g, nullptr);
+                return fail(h, QCBF_ERR_ARG, "tile shape %ld does not tile the lattice", value);
+            }
+            return QCBF_OK;
+        }
         default:
             return fail(h, QCBF_ERR_ARG, "unknown tuning key %d", key);
     }
+}
+
+static long tile_code(const qcb::lattice& g) {
+    if (g.lg_tile[0] < 0) return -1;
+    return g.lg_tile[0] + 8L * g.lg_tile[1] + 64L * g.lg_tile[2] + 512L * g.lg_tile[3];
 }
 
 extern "C" int qcbf_get_tune(const qcbf_flow* h, int key, long* value) {
@@ -647,6 +665,7 @@ extern "C" int qcbf_get_tune(const qcbf_flow* h, int key, long* value) {
     switch (key) {
         case QCBF_TUNE_MAP_MODE: *value = h->g.map_mode; return QCBF_OK;
         case QCBF_TUNE_MAP_WIN: *value = h->g.map_win; return QCBF_OK;
+        case QCBF_TUNE_TILE_SHAPE: *value = tile_code(h->g); return QCBF_OK;
         default: return fail(nullptr, QCBF_ERR_ARG, "unknown tuning key %d", key);
     }
 }
@@ -746,9 +765,23 @@ extern "C" int qcbf_autotune(qcbf_flow* h, double eps, int trial_steps, long* mo
     const int cur0 = h->cur;
     const bool timing0 = h->timing;
     h->timing = false;
-    const std::pair<int, unsigned> cands[] = {{0, 64},   {1, 64},   {2, 64},   {2, 128},
-                                              {2, 256},  {2, 512},  {2, 1024}, {2, 2048}};
-    std::pair<int, unsigned> best = {h->g.map_mode, h->g.map_win};
+    struct cand {
+        int mode;
+        unsigned win;
+        int lg[4];  // lg[0] < 0: 64 consecutive sites
+    };
+    std::vector<cand> cands;
+    const std::pair<int, unsigned> maps[] = {{0, 64}, {1, 64}, {2, 256}, {2, 1024}, {2, 4096}};
+    const int shapes[][4] = {{-1, 0, 0, 0}, {2, 2, 1, 1}, {3, 2, 1, 0},
+                             {3, 3, 0, 0},  {4, 2, 0, 0}, {2, 2, 2, 0}};
+    for (const auto& m : maps)
+        for (const auto& sh : shapes) {
+            qcb::lattice probe = h->g;
+            if (sh[0] >= 0 && !qcb::set_tile_shape(probe, sh)) continue;
+            cands.push_back({m.first, m.second, {sh[0], sh[1], sh[2], sh[3]}});
+        }
+    const qcb::lattice g0 = h->g;
+    cand best = {g0.map_mode, g0.map_win, {g0.lg_tile[0], g0.lg_tile[1], g0.lg_tile[2], g0.lg_tile[3]}};
     float best_ms = 1e30f;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = QCBF_OK;
@@ -758,8 +791,9 @@ extern "C" int qcbf_autotune(qcbf_flow* h, double eps, int trial_steps, long* mo
         rc = fail(h, QCBF_ERR_HIP, "autotune setup: %s", hipGetErrorString(e));
     for (const auto& c : cands) {
         if (rc) break;
-        h->g.map_mode = c.first;
-        h->g.map_win = c.second;
+        h->g.map_mode = c.mode;
+        h->g.map_win = c.win;
+        qcb::set_tile_shape(h->g, c.lg[0] < 0 ? nullptr : c.lg);
         rc = rk3_once(h, eps);  // warm
         if (rc) break;
         if ((e = hipEventRecord(e0, h->stream)) != hipSuccess) {
@@ -780,8 +814,9 @@ extern "C" int qcbf_autotune(qcbf_flow* h, double eps, int trial_steps, long* mo
             best = c;
         }
     }
-    h->g.map_mode = best.first;
-    h->g.map_win = best.second;
+    h->g.map_mode = best.mode;
+    h->g.map_win = best.win;
+    qcb::set_tile_shape(h->g, best.lg[0] < 0 ? nullptr : best.lg);
     h->cur = cur0;
     h->timing = timing0;
     if ((e = hipMemcpyAsync(h->U[cur0], saveU, ub, hipMemcpyDeviceToDevice, h->stream)) != hipSuccess ||
@@ -792,7 +827,7 @@ extern "C" int qcbf_autotune(qcbf_flow* h, double eps, int trial_steps, long* mo
     (void)hipFree(saveA);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
-    if (mode) *mode = best.first;
-    if (win) *win = best.second;
+    if (mode) *mode = best.mode;
+    if (win) *win = best.win;
     return rc;
 }

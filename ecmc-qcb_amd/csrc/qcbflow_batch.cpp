@@ -42,8 +42,9 @@ struct qcbf_batch {
     unsigned* d_slot = nullptr;
     int* dwarn = nullptr;
     void* stage = nullptr;
-    qcb::batch_launch bl{0, 0, 64};
+    qcb::batch_launch bl{0, 2, 1024};
     int blocks_per_cu = 2;
+    long tile_shape = -1;
     int cus = 256;
     bool use_graph = false;
     // graph of one (measure, RK3 step) pair, keyed by (cur, eps)
@@ -467,6 +468,17 @@ extern "C" int qcbf_batch_tune(qcbf_batch* b, int key, long value) {
             if (value < 1 || value > 16) return bfail(b, QCBF_ERR_ARG, "blocks per CU %ld not in 1..16", value);
             b->blocks_per_cu = (int)value;
             break;
+        case QCBF_TUNE_TILE_SHAPE: {
+            if (value != -1 && (value < 0 || value >= 4096))
+                return bfail(b, QCBF_ERR_ARG, "tile shape %ld out of range", value);
+            const int lg[4] = {(int)(value & 7), (int)((value >> 3) & 7), (int)((value >> 6) & 7),
+                               (int)((value >> 9) & 7)};
+            for (auto& cd : b->cfgs) qcb::set_tile_shape(cd.g, value == -1 ? nullptr : lg);
+            b->tile_shape = value;
+            BCHK(b, hipMemcpy(b->d_cfgs, b->cfgs.data(), sizeof(cfg_desc) * b->n,
+                              hipMemcpyHostToDevice));
+            break;
+        }
         default: return bfail(b, QCBF_ERR_ARG, "unknown tuning key %d", key);
     }
     set_grid(b);
@@ -479,6 +491,7 @@ extern "C" int qcbf_batch_get_tune(const qcbf_batch* b, int key, long* value) {
         case QCBF_TUNE_MAP_MODE: *value = b->bl.map_mode; return QCBF_OK;
         case QCBF_TUNE_MAP_WIN: *value = b->bl.map_win; return QCBF_OK;
         case QCBF_TUNE_BLOCKS_PER_CU: *value = b->blocks_per_cu; return QCBF_OK;
+        case QCBF_TUNE_TILE_SHAPE: *value = b->tile_shape; return QCBF_OK;
         default: return QCBF_ERR_ARG;
     }
 }
@@ -526,12 +539,22 @@ extern "C" int qcbf_batch_autotune(qcbf_batch* b, double eps, int trial_steps, l
         int mode;
         unsigned win;
         int bpc;
+        long shape;
     };
     std::vector<cand> cands;
-    const std::pair<int, unsigned> maps[] = {{0, 64}, {1, 64}, {2, 64}, {2, 256}, {2, 1024}};
+    const std::pair<int, unsigned> maps[] = {{0, 64}, {2, 256}, {2, 1024}};
     for (auto m : maps)
-        for (int bpc : {1, 2, 3, 4, 6, 8}) cands.push_back({m.first, m.second, bpc});
-    cand best = {b->bl.map_mode, b->bl.map_win, b->blocks_per_cu};
+        for (int bpc : {1, 2, 4})
+            for (long sh : {-1L, 594L, 83L}) cands.push_back({m.first, m.second, bpc, sh});
+    cand best = {b->bl.map_mode, b->bl.map_win, b->blocks_per_cu, b->tile_shape};
+    auto apply_shape = [&](long code) -> hipError_t {
+        const int lg[4] = {(int)(code & 7), (int)((code >> 3) & 7), (int)((code >> 6) & 7),
+                           (int)((code >> 9) & 7)};
+        for (auto& cd : b->cfgs) qcb::set_tile_shape(cd.g, code == -1 ? nullptr : lg);
+        b->tile_shape = code;
+        return hipMemcpyAsync(b->d_cfgs, b->cfgs.data(), sizeof(cfg_desc) * b->n,
+                              hipMemcpyHostToDevice, b->stream);
+    };
     float best_ms = 1e30f;
     if ((e = hipMemcpyAsync(saveU, b->U[cur0], ub, hipMemcpyDeviceToDevice, b->stream)) != hipSuccess ||
         (e = hipMemcpyAsync(saveA, b->acc, ab, hipMemcpyDeviceToDevice, b->stream)) != hipSuccess ||
@@ -542,6 +565,10 @@ extern "C" int qcbf_batch_autotune(qcbf_batch* b, double eps, int trial_steps, l
         b->bl.map_win = cands[k].win;
         b->blocks_per_cu = cands[k].bpc;
         set_grid(b);
+        if ((e = apply_shape(cands[k].shape)) != hipSuccess) {
+            rc = bfail(b, QCBF_ERR_HIP, "autotune shape: %s", hipGetErrorString(e));
+            break;
+        }
         rc = enqueue_step(b, eps, b->cur);  // warm
         if (rc) break;
         b->cur ^= 1;
@@ -570,6 +597,8 @@ extern "C" int qcbf_batch_autotune(qcbf_batch* b, double eps, int trial_steps, l
     b->bl.map_win = best.win;
     b->blocks_per_cu = best.bpc;
     set_grid(b);
+    if ((e = apply_shape(best.shape)) != hipSuccess && rc == QCBF_OK)
+        rc = bfail(b, QCBF_ERR_HIP, "autotune shape: %s", hipGetErrorString(e));
     b->cur = cur0;
     if ((e = hipMemcpyAsync(b->U[cur0], saveU, ub, hipMemcpyDeviceToDevice, b->stream)) != hipSuccess ||
         (e = hipMemcpyAsync(b->acc, saveA, ab, hipMemcpyDeviceToDevice, b->stream)) != hipSuccess ||

@@ -171,7 +171,12 @@ enum qcbf_tune_key {
                                0 linear, 1 one contiguous chunk per XCD,
                                2 XCD chunks inside windows of MAP_WIN workgroups */
     QCBF_TUNE_MAP_WIN = 1,  /* window for mode 2: a multiple of 8, >= 8 */
-    QCBF_TUNE_BLOCKS_PER_CU = 2 /* ensembles only: persistent-grid workgroups per CU */
+    QCBF_TUNE_BLOCKS_PER_CU = 2, /* ensembles only: persistent-grid workgroups per CU */
+    QCBF_TUNE_TILE_SHAPE = 3     /* sites of one flow workgroup: -1 = 64 consecutive
+                                    sites, else a 4-D box of extents 2^a x 2^b x 2^c x 2^d
+                                    (a+b+c+d = 6) encoded a + 8b + 64c + 512d; the box
+                                    must divide the lattice (ensembles: applied to each
+                                    member it divides) */
 };
 /* Set one launch parameter of h; results do not depend on it (only the order in
  * which workgroups visit sites changes).  QCBF_ERR_ARG for an unknown key or
@@ -179,9 +184,10 @@ enum qcbf_tune_key {
 int qcbf_tune(qcbf_flow* h, int key, long value);
 int qcbf_get_tune(const qcbf_flow* h, int key, long* value);
 
-/* Time each candidate launch shape of the flow kernels for trial_steps RK3
- * steps on a copy of the current field (the field is restored afterwards) and
- * keep the fastest; *mode and *win (may be NULL) receive the choice. */
+/* Time each candidate launch shape of the flow kernels (tile order x 4-D tile
+ * shape, see qcbf_tune_key) for trial_steps RK3 steps on the real field, which
+ * is saved first and restored afterwards, and keep the fastest; *mode and *win
+ * (may be NULL) receive the chosen order, qcbf_get_tune the tile shape. */
 int qcbf_autotune(qcbf_flow* h, double eps, int trial_steps, long* mode, long* win);
 
 /* ---- ensembles (no counterpart in the reference: its callers flow one
@@ -211,8 +217,8 @@ int qcbf_batch_synchronize(qcbf_batch* b);
 int qcbf_batch_tune(qcbf_batch* b, int key, long value);
 int qcbf_batch_get_tune(const qcbf_batch* b, int key, long* value);
 int qcbf_batch_use_graph(qcbf_batch* b, int on);
-/* Autotuner: tile order x persistent-grid size, timed on a copy of the fields
- * (restored afterwards); the fastest is kept. */
+/* Autotuner: tile order x persistent-grid size x tile shape, timed on the real
+ * fields (saved first, restored afterwards); the fastest is kept. */
 int qcbf_batch_autotune(qcbf_batch* b, double eps, int trial_steps, long* mode, long* win,
                         long* blocks_per_cu);
 /* Returns the device time of the flow_measure calls since the previous call

@@ -28,11 +28,13 @@ def fields_for(dims, sweeps=3):
 @pytest.mark.parametrize("graph", [False, True])
 def test_batch_equals_single_lattice_path(prec, graph):
     """Each member of the ensemble gets bit-for-bit the series the
-    single-lattice path gives (same per-link code, same reduction order)."""
+    single-lattice path gives (same per-link code, same reduction order),
+    whatever the tile shape of the members it divides."""
     say(f"ensemble of {len(DIMS)} uneven lattices, precision {prec}, graph {graph}")
     fs = fields_for(DIMS)
     with qcbflow.Ensemble(DIMS, 0.02, prec) as ens:
         ens.use_graph(graph)
+        ens.tune(qcbflow.TUNE_TILE_SHAPE, qcbflow.tile_code(4, 4, 2, 2) if graph else -1)
         for c, f in enumerate(fs):
             ens.copy(c, f)
         tqe, plaq = ens.flow_measure(9, 1)
@@ -106,11 +108,18 @@ def test_single_lattice_autotune_and_tile_maps_agree():
         m, w = gf.autotune(trial_steps=1)
         assert np.array_equal(gf.field(), f)
         results = []
-        for mode, win in [(0, 64), (1, 64), (2, 64), (2, 1024)]:
+        shapes = [-1, qcbflow.tile_code(4, 4, 2, 2), qcbflow.tile_code(8, 4, 2, 1),
+                  qcbflow.tile_code(8, 8, 1, 1), qcbflow.tile_code(2, 2, 4, 4)]
+        for mode, win, shape in [(0, 64, -1), (1, 64, shapes[1]), (2, 64, shapes[2]),
+                                 (2, 1024, shapes[3]), (0, 64, shapes[4])]:
             gf.tune(qcbflow.TUNE_MAP_MODE, mode)
             gf.tune(qcbflow.TUNE_MAP_WIN, win)
+            gf.tune(qcbflow.TUNE_TILE_SHAPE, shape)
+            assert gf.get_tune(qcbflow.TUNE_TILE_SHAPE) == shape
             gf.copy(f)
             results.append(gf.flow_measure(4, 1))
+        with pytest.raises(qcbflow.QcbfError):  # 32 sites along x do not tile a 16-site extent
+            gf.tune(qcbflow.TUNE_TILE_SHAPE, qcbflow.tile_code(32, 2, 1, 1))
     for r in results[1:]:
         assert np.array_equal(r[0], results[0][0]) and np.array_equal(r[1], results[0][1])
 
