@@ -80,6 +80,7 @@ hipError_t launch_batch_stage(prec p, int stage, const void* Uin, void* Uout, vo
                               const cfg_desc* cfgs, const unsigned* tile_cfg, unsigned ntiles,
                               const batch_launch& bl, double cz, double ca, int* warn,
                               hipStream_t st) {
+    (void)hipGetLastError();  // drop a stale error of an earlier call
     return p == prec::f64
                ? batch_stage_launch<prec::f64>(stage, Uin, Uout, acc, cfgs, tile_cfg, ntiles, bl,
                                                cz, ca, warn, st)
@@ -154,6 +155,7 @@ __global__ void k_bump(unsigned* ctr) { *ctr += 1u; }
 __global__ void k_set(unsigned* ctr, unsigned v) { *ctr = v; }
 
 hipError_t launch_set_counter(unsigned* ctr, unsigned value, hipStream_t st) {
+    (void)hipGetLastError();  // drop a stale error of an earlier call
     k_set<<<1, 1, 0, st>>>(ctr, value);
     return hipGetLastError();
 }
@@ -162,6 +164,7 @@ hipError_t launch_batch_measure(prec p, const void* U, const cfg_desc* cfgs,
                                 const unsigned* tile_cfg, unsigned ntiles, int ncfg,
                                 const batch_launch& bl, double* partials, double* out,
                                 unsigned* slot_ctr, bool bump, hipStream_t st) {
+    (void)hipGetLastError();  // drop a stale error of an earlier call
     const dim3 block(kSitesPerBlock, 3);
     if (p == prec::f64)
         k_batch_measure<prec::f64><<<bl.grid, block, 0, st>>>(

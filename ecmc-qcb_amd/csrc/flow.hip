@@ -74,6 +74,7 @@ static hipError_t stage_launch(int stage, const void* Uin, void* Uout, void* acc
 
 hipError_t launch_rk3_stage(prec p, int stage, const void* Uin, void* Uout, void* acc,
                             const lattice& g, double cz, double ca, int* warn, hipStream_t st) {
+    (void)hipGetLastError();  // drop a stale error of an earlier call
     return p == prec::f64 ? stage_launch<prec::f64>(stage, Uin, Uout, acc, g, cz, ca, warn, st)
                           : stage_launch<prec::f32m>(stage, Uin, Uout, acc, g, cz, ca, warn, st);
 }
@@ -170,12 +171,14 @@ static hipError_t unfused_launch(int which, void* U, void* F0, const void* F1, c
 }
 
 hipError_t launch_force(prec p, const void* U, void* F, const lattice& g, hipStream_t st) {
+    (void)hipGetLastError();  // drop a stale error of an earlier call
     void* u = const_cast<void*>(U);
     return p == prec::f64 ? unfused_launch<prec::f64>(0, u, F, nullptr, g, 0.0, nullptr, st)
                           : unfused_launch<prec::f32m>(0, u, F, nullptr, g, 0.0, nullptr, st);
 }
 hipError_t launch_update(prec p, int which, void* U, const void* F0, const void* F1,
                          const lattice& g, double eps, int* warn, hipStream_t st) {
+    (void)hipGetLastError();  // drop a stale error of an earlier call
     if (which < 1 || which > 3) return hipErrorInvalidValue;
     void* f0 = const_cast<void*>(F0);
     return p == prec::f64 ? unfused_launch<prec::f64>(which, U, f0, F1, g, eps, warn, st)
@@ -183,6 +186,7 @@ hipError_t launch_update(prec p, int which, void* U, const void* F0, const void*
 }
 hipError_t launch_replace_force0(prec p, void* F0, const void* F1, const lattice& g,
                                  hipStream_t st) {
+    (void)hipGetLastError();  // drop a stale error of an earlier call
     return p == prec::f64 ? unfused_launch<prec::f64>(4, nullptr, F0, F1, g, 0.0, nullptr, st)
                           : unfused_launch<prec::f32m>(4, nullptr, F0, F1, g, 0.0, nullptr, st);
 }

@@ -157,6 +157,7 @@ __global__ __launch_bounds__(256) void k_project_su3(double2* __restrict__ U, un
 
 hipError_t launch_heatbath_f64(double2* U, const lattice& g, double beta, int n_sweeps,
                                uint64_t seed, unsigned sweep0, int* stuck, hipStream_t st) {
+    (void)hipGetLastError();  // drop a stale error of an earlier call
     const unsigned nb = (g.V + 255) / 256;
     for (int sw = 0; sw < n_sweeps; sw++) {
         for (int mu = 0; mu < 4; mu++)

@@ -159,19 +159,32 @@ __host__ __device__ inline unsigned stage_tiles(const lattice& g) {
 }
 
 // ---- FP64 structure-of-arrays link field -----------------------------------
+#ifndef QCB_F64_STREAM_ROWS
+#define QCB_F64_STREAM_ROWS 1
+#endif
+#ifndef QCB_F64_RECONSTRUCT
+#define QCB_F64_RECONSTRUCT 0
+#endif
 struct field64 {
     using real = double;
-    static constexpr bool stream_rows = true;
+    // Staple products stream one operand from memory a row at a time.
+    static constexpr bool stream_rows = QCB_F64_STREAM_ROWS && !QCB_F64_RECONSTRUCT;
     const double2* __restrict__ p;
     unsigned V;
     __device__ inline mat3<double> link(int mu, unsigned s) const {
         mat3<double> m;
         const double2* b = p + (size_t)mu * 9 * V + s;
 #pragma unroll
-        for (int k = 0; k < 9; k++) {
+        for (int k = 0; k < (QCB_F64_RECONSTRUCT ? 6 : 9); k++) {
             double2 v = b[(size_t)k * V];
             m.e[k / 3][k % 3] = {v.x, v.y};
         }
+#if QCB_F64_RECONSTRUCT
+        // third row of an SU(3) matrix: conj(row0 x row1)
+        m.e[2][0] = conj(m.e[0][1] * m.e[1][2] - m.e[0][2] * m.e[1][1]);
+        m.e[2][1] = conj(m.e[0][2] * m.e[1][0] - m.e[0][0] * m.e[1][2]);
+        m.e[2][2] = conj(m.e[0][0] * m.e[1][1] - m.e[0][1] * m.e[1][0]);
+#endif
         return m;
     }
     // row i of U_mu(s)
@@ -185,11 +198,26 @@ struct field64 {
     }
 };
 
+#ifndef QCB_NT_STORES
+#define QCB_NT_STORES 0
+#endif
+// Stores of the flow's output fields; with QCB_NT_STORES they bypass the caches
+// (the next stage reads them from HBM anyway).
+__device__ inline void put2(double2* p, double2 v) {
+#if QCB_NT_STORES
+    __builtin_nontemporal_store(v.x, &p->x);
+    __builtin_nontemporal_store(v.y, &p->y);
+#else
+    *p = v;
+#endif
+}
+
 __device__ inline void store_link64(double2* __restrict__ p, unsigned V, int mu, unsigned s,
                                     const mat3<double>& m) {
     double2* b = p + (size_t)mu * 9 * V + s;
 #pragma unroll
-    for (int k = 0; k < 9; k++) b[(size_t)k * V] = make_double2(m.e[k / 3][k % 3].r, m.e[k / 3][k % 3].i);
+    for (int k = 0; k < 9; k++)
+        put2(b + (size_t)k * V, make_double2(m.e[k / 3][k % 3].r, m.e[k / 3][k % 3].i));
 }
 
 __device__ inline alg8<double> load_alg64(const double2* __restrict__ p, unsigned V, int mu,
@@ -209,7 +237,7 @@ __device__ inline void store_alg64(double2* __restrict__ p, unsigned V, int mu, 
                                    const alg8<double>& a) {
     double2* b = p + (size_t)mu * 4 * V + s;
 #pragma unroll
-    for (int k = 0; k < 4; k++) b[(size_t)k * V] = make_double2(a.a[2 * k], a.a[2 * k + 1]);
+    for (int k = 0; k < 4; k++) put2(b + (size_t)k * V, make_double2(a.a[2 * k], a.a[2 * k + 1]));
 }
 
 }  // namespace qcb

@@ -118,6 +118,7 @@ __global__ __launch_bounds__(256) void k_field_to_aos(const typename storage<P>:
 
 hipError_t launch_aos_to_field(prec p, const void* aos, int kind, void* U, const lattice& g,
                                unsigned site0, unsigned nsites, hipStream_t st) {
+    (void)hipGetLastError();  // drop a stale error of an earlier call
     if (nsites == 0) return hipSuccess;
     const unsigned nb = (nsites + kTile - 1) / kTile;
     if (p == prec::f64)
@@ -131,6 +132,7 @@ hipError_t launch_aos_to_field(prec p, const void* aos, int kind, void* U, const
 
 hipError_t launch_field_to_aos(prec p, const void* U, double* aos, const lattice& g,
                                unsigned site0, unsigned nsites, bool bswap, hipStream_t st) {
+    (void)hipGetLastError();  // drop a stale error of an earlier call
     if (nsites == 0) return hipSuccess;
     const unsigned nb = (nsites + kTile - 1) / kTile;
     if (p == prec::f64)
@@ -158,6 +160,7 @@ __global__ __launch_bounds__(256) void k_alg_to_aos(const typename storage<P>::a
 }
 hipError_t launch_alg_to_aos(prec p, const void* alg, double* aos, const lattice& g,
                              hipStream_t st) {
+    (void)hipGetLastError();  // drop a stale error of an earlier call
     const unsigned nb = (g.V + kSitesPerBlock - 1) / kSitesPerBlock;
     if (p == prec::f64)
         k_alg_to_aos<prec::f64><<<nb, dim3(kSitesPerBlock, 4), 0, st>>>(
@@ -238,6 +241,7 @@ __global__ __launch_bounds__(256) void k_cold_start(typename storage<P>::vec* __
 }
 
 hipError_t launch_hot_start(prec p, void* U, const lattice& g, uint64_t seed, hipStream_t st) {
+    (void)hipGetLastError();  // drop a stale error of an earlier call
     const unsigned nb = (g.V + kSitesPerBlock - 1) / kSitesPerBlock;
     if (p == prec::f64)
         k_hot_start<prec::f64><<<nb, dim3(kSitesPerBlock, 4), 0, st>>>(static_cast<double2*>(U),
@@ -248,6 +252,7 @@ hipError_t launch_hot_start(prec p, void* U, const lattice& g, uint64_t seed, hi
     return hipGetLastError();
 }
 hipError_t launch_cold_start(prec p, void* U, const lattice& g, hipStream_t st) {
+    (void)hipGetLastError();  // drop a stale error of an earlier call
     const unsigned nb = (g.V + kSitesPerBlock - 1) / kSitesPerBlock;
     if (p == prec::f64)
         k_cold_start<prec::f64><<<nb, dim3(kSitesPerBlock, 4), 0, st>>>(static_cast<double2*>(U),
@@ -287,6 +292,7 @@ __global__ __launch_bounds__(256) void k_exp_batch64(const double* __restrict__ 
 
 hipError_t launch_exp_batch_f64(const double* Z, double* out, int n, double coeff, int* warn,
                                 hipStream_t st) {
+    (void)hipGetLastError();  // drop a stale error of an earlier call
     if (n <= 0) return hipSuccess;
     k_exp_batch64<<<(n + 255) / 256, 256, 0, st>>>(Z, out, n, coeff, warn);
     return hipGetLastError();
@@ -301,6 +307,7 @@ __global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ src,
 }
 
 hipError_t launch_copy(const double2* src, double2* dst, size_t n, hipStream_t st) {
+    (void)hipGetLastError();  // drop a stale error of an earlier call
     const size_t want = (n + 255) / 256;
     const unsigned grid = (unsigned)(want < 65536 ? want : 65536);
     k_copy<<<grid ? grid : 1, 256, 0, st>>>(src, dst, n);

@@ -82,6 +82,7 @@ static hipError_t measure_launch(const void* U, const lattice& g, double* partia
 
 hipError_t launch_measure(prec p, const void* U, const lattice& g, double* partials, double* out,
                           bool clover, hipStream_t st) {
+    (void)hipGetLastError();  // drop a stale error of an earlier call
     return p == prec::f64 ? measure_launch<prec::f64>(U, g, partials, out, clover, st)
                           : measure_launch<prec::f32m>(U, g, partials, out, clover, st);
 }
@@ -157,6 +158,7 @@ __global__ __launch_bounds__(256) void k_reduce_max2(const double* __restrict__ 
 
 hipError_t launch_unitarity(prec p, const void* U, const lattice& g, double* partials,
                             double* out, hipStream_t st) {
+    (void)hipGetLastError();  // drop a stale error of an earlier call
     const unsigned nblk = measure_blocks(g);
     if (p == prec::f64)
         k_unitarity<prec::f64><<<nblk, dim3(kSitesPerBlock, 4), 0, st>>>(

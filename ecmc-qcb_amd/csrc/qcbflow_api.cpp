@@ -73,9 +73,11 @@ static int fail(qcbf_flow* h, int code, const char* fmt, ...) {
 #define HIPCHK(h, expr)                                                                     \
     do {                                                                                    \
         hipError_t e_ = (expr);                                                             \
-        if (e_ != hipSuccess)                                                               \
+        if (e_ != hipSuccess) {                                                             \
+            (void)hipGetLastError(); /* the error is reported here, not by the next call */  \
             return fail((h), e_ == hipErrorOutOfMemory ? QCBF_ERR_OOM : QCBF_ERR_HIP,       \
                         "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+        }                                                                                   \
     } while (0)
 
 #define CHECK_H(h)                                                    \
@@ -175,6 +177,7 @@ extern "C" int qcbf_create(const int dims[4], int precision, qcbf_flow** out) {
     do {                                                                                      \
         hipError_t e2 = (expr);                                                               \
         if (e2 != hipSuccess) {                                                               \
+            (void)hipGetLastError();                                                          \
             release(h);                                                                       \
             return fail(nullptr, e2 == hipErrorOutOfMemory ? QCBF_ERR_OOM : QCBF_ERR_HIP,     \
                         "%s: %s", #expr, hipGetErrorString(e2));                              \

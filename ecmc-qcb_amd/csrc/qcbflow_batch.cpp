@@ -72,9 +72,11 @@ static thread_local std::string g_batch_err;
 #define BCHK(b, expr)                                                                      \
     do {                                                                                   \
         hipError_t e_ = (expr);                                                            \
-        if (e_ != hipSuccess)                                                              \
+        if (e_ != hipSuccess) {                                                            \
+            (void)hipGetLastError();                                                       \
             return bfail((b), e_ == hipErrorOutOfMemory ? QCBF_ERR_OOM : QCBF_ERR_HIP,     \
                          "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+        }                                                                                  \
     } while (0)
 
 #define CHECK_B(b)                                                              \
@@ -176,6 +178,7 @@ extern "C" int qcbf_batch_create(int n, const int* dims, int precision, qcbf_bat
     b->alg_words = ao;
     const size_t wbytes = precision == QCBF_FP64 ? sizeof(double2) : sizeof(float4);
     auto fail_create = [&](hipError_t e, const char* what) {
+        (void)hipGetLastError();
         g_batch_err = std::string(what) + ": " + hipGetErrorString(e);
         const int code = e == hipErrorOutOfMemory ? QCBF_ERR_OOM : QCBF_ERR_HIP;
         release(b);
