@@ -46,6 +46,17 @@ struct field32c {
     }
 };
 
+__device__ inline void put4(float4* p, float4 v) {
+#if QCB_NT_STORES
+    __builtin_nontemporal_store(v.x, &p->x);
+    __builtin_nontemporal_store(v.y, &p->y);
+    __builtin_nontemporal_store(v.z, &p->z);
+    __builtin_nontemporal_store(v.w, &p->w);
+#else
+    *p = v;
+#endif
+}
+
 // FP64 Gram-Schmidt of two rows: r0 <- r0/|r0|, r1 <- (r1 - <r0,r1> r0)/|.|
 __device__ inline void orthonormalise_rows(cpx<double> (&r0)[3], cpx<double> (&r1)[3]) {
     double n0 = norm2(r0[0]) + norm2(r0[1]) + norm2(r0[2]);
@@ -67,9 +78,10 @@ __device__ inline void store_rows32c(float4* __restrict__ p, unsigned V, int mu,
                                      cpx<double> (&r0)[3], cpx<double> (&r1)[3]) {
     orthonormalise_rows(r0, r1);
     float4* b = p + (size_t)mu * 3 * V + s;
-    b[0] = make_float4((float)r0[0].r, (float)r0[0].i, (float)r0[1].r, (float)r0[1].i);
-    b[V] = make_float4((float)r0[2].r, (float)r0[2].i, (float)r1[0].r, (float)r1[0].i);
-    b[2 * (size_t)V] = make_float4((float)r1[1].r, (float)r1[1].i, (float)r1[2].r, (float)r1[2].i);
+    put4(b, make_float4((float)r0[0].r, (float)r0[0].i, (float)r0[1].r, (float)r0[1].i));
+    put4(b + V, make_float4((float)r0[2].r, (float)r0[2].i, (float)r1[0].r, (float)r1[0].i));
+    put4(b + 2 * (size_t)V,
+         make_float4((float)r1[1].r, (float)r1[1].i, (float)r1[2].r, (float)r1[2].i));
 }
 
 template <typename T>
@@ -93,8 +105,8 @@ __device__ inline alg8<float> load_alg32(const float4* __restrict__ p, unsigned 
 __device__ inline void store_alg32(float4* __restrict__ p, unsigned V, int mu, unsigned s,
                                    const alg8<float>& a) {
     float4* b = p + (size_t)mu * 2 * V + s;
-    b[0] = make_float4(a.a[0], a.a[1], a.a[2], a.a[3]);
-    b[V] = make_float4(a.a[4], a.a[5], a.a[6], a.a[7]);
+    put4(b, make_float4(a.a[0], a.a[1], a.a[2], a.a[3]));
+    put4(b + V, make_float4(a.a[4], a.a[5], a.a[6], a.a[7]));
 }
 
 // ---- storage traits used by the generic kernels ------------------------------

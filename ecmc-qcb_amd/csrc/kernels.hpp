@@ -15,7 +15,7 @@ namespace qcb {
 constexpr int kSitesPerBlock = 64;
 
 // Number of per-workgroup partial triples written by the measurement kernel.
-inline unsigned measure_blocks(const lattice& g) {
+__host__ __device__ inline unsigned measure_blocks(const lattice& g) {
     return (g.V + kSitesPerBlock - 1) / kSitesPerBlock;
 }
 

@@ -159,15 +159,20 @@ __host__ __device__ inline unsigned stage_tiles(const lattice& g) {
 }
 
 // ---- FP64 structure-of-arrays link field -----------------------------------
+// The flow and measurement kernels read only the first two rows of every
+// FP64 link and rebuild the third as conj(row0 x row1), exact for SU(3) up to
+// rounding: a third fewer cache lines per link, which is what bounds these
+// kernels on MI355X (the field in HBM keeps all nine entries).  With
+// QCB_F64_RECONSTRUCT=0 all nine are read, and one operand of every staple
+// product is streamed a row at a time (QCB_F64_STREAM_ROWS).
 #ifndef QCB_F64_STREAM_ROWS
 #define QCB_F64_STREAM_ROWS 1
 #endif
 #ifndef QCB_F64_RECONSTRUCT
-#define QCB_F64_RECONSTRUCT 0
+#define QCB_F64_RECONSTRUCT 1
 #endif
 struct field64 {
     using real = double;
-    // Staple products stream one operand from memory a row at a time.
     static constexpr bool stream_rows = QCB_F64_STREAM_ROWS && !QCB_F64_RECONSTRUCT;
     const double2* __restrict__ p;
     unsigned V;
@@ -199,10 +204,11 @@ struct field64 {
 };
 
 #ifndef QCB_NT_STORES
-#define QCB_NT_STORES 0
+#define QCB_NT_STORES 1
 #endif
-// Stores of the flow's output fields; with QCB_NT_STORES they bypass the caches
-// (the next stage reads them from HBM anyway).
+// Stores of the flow's output fields are non-temporal: the next stage reads
+// them back from HBM anyway, and keeping them out of L2 leaves it to the
+// neighbour links the current stage re-reads.
 __device__ inline void put2(double2* p, double2 v) {
 #if QCB_NT_STORES
     __builtin_nontemporal_store(v.x, &p->x);

@@ -24,7 +24,10 @@ __global__ __launch_bounds__(192) void k_measure(const typename storage<P>::vec*
                                                  lattice g, double* __restrict__ partials) {
     using FT = typename storage<P>::field;
     __shared__ double lds[3 * 3];
-    const unsigned s = blockIdx.x * kSitesPerBlock + threadIdx.x;
+    const unsigned ntiles = measure_blocks(g);
+    const unsigned tile = tile_of_block(blockIdx.x, ntiles, g.map_mode, g.map_win);
+    if (tile >= ntiles) return;  // a whole padding workgroup: no barrier is skipped
+    const unsigned s = tile * kSitesPerBlock + threadIdx.x;
     double v[3] = {0.0, 0.0, 0.0};
     if (s < g.V) {
         const nbr n = neighbours(g, s);
@@ -37,9 +40,9 @@ __global__ __launch_bounds__(192) void k_measure(const typename storage<P>::vec*
     }
     block_sum<3>(v, lds, 3);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
-        partials[3 * blockIdx.x + 0] = v[0];
-        partials[3 * blockIdx.x + 1] = v[1];
-        partials[3 * blockIdx.x + 2] = v[2];
+        partials[3 * tile + 0] = v[0];
+        partials[3 * tile + 1] = v[1];
+        partials[3 * tile + 2] = v[2];
     }
 }
 
@@ -70,12 +73,13 @@ template <prec P>
 static hipError_t measure_launch(const void* U, const lattice& g, double* partials, double* out,
                                  bool clover, hipStream_t st) {
     const unsigned nblk = measure_blocks(g);
+    const unsigned grid = grid_for_tiles(nblk, g.map_mode, g.map_win);
     const dim3 block(kSitesPerBlock, 3);
     const auto* u = static_cast<const typename storage<P>::vec*>(U);
     if (clover)
-        k_measure<P, true><<<nblk, block, 0, st>>>(u, g, partials);
+        k_measure<P, true><<<grid, block, 0, st>>>(u, g, partials);
     else
-        k_measure<P, false><<<nblk, block, 0, st>>>(u, g, partials);
+        k_measure<P, false><<<grid, block, 0, st>>>(u, g, partials);
     k_reduce_measure<<<1, 256, 0, st>>>(partials, nblk, 1.0 / (double)g.V, out);
     return hipGetLastError();
 }

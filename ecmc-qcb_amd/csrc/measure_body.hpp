@@ -13,6 +13,117 @@ struct fclov {
     cpx<T> o[3];  // (0,1), (0,2), (1,2)
 };
 
+// Products whose right operand is streamed from memory a row at a time:
+//   R  = A B        (rows k of B)        R += A B      (rows k of B)
+//   R  = A B^dag    (rows j of B)        R += A B^dag  (rows j of B)
+//   R  = A^dag B    (rows k of A and B)  R  = M B      (rows i of M, B whole)
+template <class F>
+__device__ inline void acc_mul_s(mat3<typename F::real>& R, const mat3<typename F::real>& A,
+                                 const F& f, int mu, unsigned s) {
+    using T = typename F::real;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        cpx<T> b[3];
+        f.row(mu, s, k, b);
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) R.e[i][j] = madd(R.e[i][j], A.e[i][k], b[j]);
+    }
+}
+template <class F>
+__device__ inline void acc_mul_nd_s(mat3<typename F::real>& R, const mat3<typename F::real>& A,
+                                    const F& f, int mu, unsigned s) {
+    using T = typename F::real;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        cpx<T> b[3];
+        f.row(mu, s, j, b);
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+            R.e[i][j] = madd_c(madd_c(madd_c(R.e[i][j], A.e[i][0], b[0]), A.e[i][1], b[1]), A.e[i][2],
+                               b[2]);
+    }
+}
+template <class F>
+__device__ inline mat3<typename F::real> mul_rows_s(const F& f, int mu, unsigned s,
+                                                    const mat3<typename F::real>& B) {
+    using T = typename F::real;
+    mat3<T> R;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        cpx<T> a[3];
+        f.row(mu, s, i, a);
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            R.e[i][j] = madd(madd(a[0] * B.e[0][j], a[1], B.e[1][j]), a[2], B.e[2][j]);
+    }
+    return R;
+}
+template <class F>
+__device__ inline mat3<typename F::real> mul_dn_s(const F& f, int mua, unsigned sa, int mub,
+                                                  unsigned sb) {
+    using T = typename F::real;
+    mat3<T> R = zero3<T>();
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        cpx<T> a[3], b[3];
+        f.row(mua, sa, k, a);
+        f.row(mub, sb, k, b);
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) R.e[i][j] = madd_cl(R.e[i][j], a[i], b[j]);
+    }
+    return R;
+}
+
+// The four clover leaves with streamed operands (FP64 storage): at most the
+// running sum, two products and one whole link are live.
+template <class F, int MU, int NU>
+__device__ inline mat3<typename F::real> clover_leaves_streamed(
+    const F& f, unsigned s, unsigned xpmu, unsigned xpnu, unsigned xmmu, unsigned xmnu,
+    unsigned xmmupnu, unsigned xmmumnu, unsigned xpmumnu, double& plaq) {
+    using T = typename F::real;
+    mat3<T> C = zero3<T>();
+    {  // U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
+        const mat3<T> a = mul_rows_s(f, MU, s, f.link(NU, xpmu));
+        mat3<T> b = zero3<T>();
+        acc_mul_nd_s(b, a, f, MU, xpnu);
+        acc_mul_nd_s(C, b, f, NU, s);
+        plaq += (double)(C.e[0][0].r + C.e[1][1].r + C.e[2][2].r);
+    }
+    {  // U_nu(x) U_mu(x-mu+nu)^dag U_nu(x-mu)^dag U_mu(x-mu)
+        mat3<T> a = zero3<T>();
+        acc_mul_nd_s(a, f.link(NU, s), f, MU, xmmupnu);
+        mat3<T> b = zero3<T>();
+        acc_mul_nd_s(b, a, f, NU, xmmu);
+        acc_mul_s(C, b, f, MU, xmmu);
+    }
+    {  // U_mu(x-mu)^dag U_nu(x-mu-nu)^dag U_mu(x-mu-nu) U_nu(x-nu)
+        //  = (U_nu(x-mu-nu) U_mu(x-mu))^dag U_mu(x-mu-nu) U_nu(x-nu)
+        const mat3<T> a = mul_rows_s(f, NU, xmmumnu, f.link(MU, xmmu));
+        mat3<T> d = zero3<T>();
+#pragma unroll
+        for (int k = 0; k < 3; k++) {  // d = a^dag U_mu(x-mu-nu)
+            cpx<T> b[3];
+            f.row(MU, xmmumnu, k, b);
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) d.e[i][j] = madd_cl(d.e[i][j], a.e[k][i], b[j]);
+        }
+        acc_mul_s(C, d, f, NU, xmnu);
+    }
+    {  // U_nu(x-nu)^dag U_mu(x-nu) U_nu(x+mu-nu) U_mu(x)^dag
+        const mat3<T> a = mul_dn_s(f, NU, xmnu, MU, xmnu);
+        mat3<T> b = zero3<T>();
+        acc_mul_s(b, a, f, NU, xpmumnu);
+        acc_mul_nd_s(C, b, f, MU, s);
+    }
+    return C;
+}
+
 // Sum of the four leaves of plane (MU,NU) at x, all anticlockwise, in the
 // order and association of clover_site (observables_mpi.cpp:78-86).  Adds
 // Re tr(leaf 0) to plaq.
@@ -25,6 +136,10 @@ __device__ inline fclov<typename F::real> clover(const F& f, unsigned s, const n
     const unsigned xmmupnu = xmmu + n.up[NU], xmmumnu = xmmu + n.dn[NU];
     const unsigned xpmumnu = xpmu + n.dn[NU];
     mat3<T> C;
+    if constexpr (F::stream_rows) {
+        C = clover_leaves_streamed<F, MU, NU>(f, s, xpmu, xpnu, xmmu, xmnu, xmmupnu, xmmumnu,
+                                              xpmumnu, plaq);
+    } else {
     {  // U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
         const mat3<T> a = mul(f.link(MU, s), f.link(NU, xpmu));
         const mat3<T> b = mul_nd(a, f.link(MU, xpnu));
@@ -45,6 +160,7 @@ __device__ inline fclov<typename F::real> clover(const F& f, unsigned s, const n
         const mat3<T> a = mul_dn(f.link(NU, xmnu), f.link(MU, xmnu));
         const mat3<T> b = mul(a, f.link(NU, xpmumnu));
         acc_mul_nd(C, b, f.link(MU, s));
+    }
     }
     fclov<T> Fc;
     const T e8 = (T)0.125;
