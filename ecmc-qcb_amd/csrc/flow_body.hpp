@@ -5,6 +5,10 @@
 
 namespace qcb {
 
+#ifndef QCB_SCHED_SPLIT
+#define QCB_SCHED_SPLIT 0
+#endif
+
 // Staple sum of U_mu(x):
 //   sum_{nu != mu} U_nu(x+mu) (U_nu(x) U_mu(x+nu))^dag
 //                + (U_mu(x-nu) U_nu(x+mu-nu))^dag U_nu(x-nu)
@@ -71,11 +75,19 @@ __device__ inline mat3<typename F::real> staple_sum(const F& f, unsigned s, cons
                 const mat3<T> a = mul(f.link(nu, s), f.link(MU, xnu));
                 acc_mul_nd(st, f.link(nu, xmu), a);
             }
+#if QCB_SCHED_SPLIT >= 2
+            __builtin_amdgcn_sched_barrier(0);
+#endif
             {
                 const mat3<T> b = mul(f.link(MU, xmnu), f.link(nu, xmunu));
                 acc_mul_dn(st, b, f.link(nu, xmnu));
             }
         }
+#if QCB_SCHED_SPLIT >= 1
+        // keep the scheduler from hoisting the next plane's loads above this
+        // one's arithmetic: fewer live registers, more resident wavefronts
+        __builtin_amdgcn_sched_barrier(0);
+#endif
     }
     return st;
 }
