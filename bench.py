@@ -1,16 +1,22 @@
 """Throughput of the Wilson-flow measurement path on one MI355X.
 
-Workload (default): a 32^4 SU(3) lattice in FP64, Wilson flow at eps = 0.02
-with the clover Q, E and the plaquette measured after every RK3 step.  One
-bench "step" = one RK3 step (three fused stage kernels) + one measurement.
-The timed region runs K steps through qcbf_flow_measure (K RK3 steps and
-K+1 measurements, one synchronisation at the end).
+Workload (default): a 32^4 SU(3) lattice in FP64, Haar hot start from seed 7
+thermalised by 50 checkerboard heatbath sweeps at beta = 6.0 on the GPU, then
+Wilson flow at eps = 0.02 with the clover Q, E and the plaquette measured
+after every RK3 step.  One bench "step" = one RK3 step (three fused stage
+kernels) + one measurement.  The timed region runs K steps (default 200)
+through qcbf_flow_measure (K RK3 steps and K+1 measurements, one
+synchronisation at the end).
 
 Prints ONE JSON line:  value = site-flow-steps/s = V * K / seconds.
 The roofline object reports the dominant kernel (the fused RK3 stage
 kernel; its three instantiations are timed together with HIP events on the
 handle's stream inside the timed region).  The CPU baseline is the FP64
 oracle (a C restatement of the reference flow) on the host cores.
+
+--report precision | ensemble | large  runs one of the other workloads
+instead (48^4 FP32-mixed vs FP64 accuracy and t0; the 64-lattice ensemble of
+uneven volumes; 64^4 FP32-mixed vs FP64) and prints one JSON line for it.
 """
 import argparse
 import json
@@ -18,6 +24,8 @@ import os
 import re
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "ecmc-qcb_amd"))
@@ -81,31 +89,271 @@ def cpu_baseline(links, L, eps, steps, threads):
     }
 
 
-def timed_flow(gf, K, warmup):
-    """K bench steps (RK3 step + measurement) on an already filled handle.
+def timed_flow(h, src, K, warmup, every=1):
+    """Warmup on the handle, then refill it from `src` (a handle holding the
+    input field) so the timed flow starts at t = 0 on the input, and run
+    K RK3 steps with a measurement every `every` steps (K divisible by it).
     Returns (seconds, tQE, plaquettes, per-class kernel ms, per-class launches)."""
     if warmup > 0:
-        gf.flow_measure(warmup + 1, 1)
-    gf.synchronize()
-    gf.timing(True)
-    gf.synchronize()
+        h.flow_measure(warmup + 1, 1)
+    h.copy_from(src)
+    h.synchronize()
+    h.timing(True)
+    h.synchronize()
     t0 = time.perf_counter()
-    tqe, plaq = gf.flow_measure(K + 1, 1)
-    gf.synchronize()
+    tqe, plaq = h.flow_measure(K // every + 1, every)
+    h.synchronize()
     secs = time.perf_counter() - t0
-    ms, launches = gf.timing_read()
-    gf.timing(False)
+    ms, launches = h.timing_read()
+    h.timing(False)
     return secs, tqe, plaq, ms, launches
+
+
+def thermalised(qcbflow, L, beta, seed, sweeps):
+    """FP64 handle: Haar hot start from `seed`, then `sweeps` heatbath sweeps."""
+    g = qcbflow.GradientFlow(L)
+    g.hot_start(seed)
+    if sweeps > 0:
+        g.heatbath(beta, sweeps, seed)
+    g.synchronize()
+    return g
+
+
+def t_at(tqe, target):
+    """Flow time where t^2 E(t) first reaches `target` (linear interpolation
+    between measurements), or None when the flow stops short of it."""
+    t, E = tqe[:, 0], tqe[:, 2]
+    y = t * t * E
+    k = np.nonzero(y >= target)[0]
+    if len(k) == 0 or k[0] == 0:
+        return None
+    k = k[0]
+    return float(t[k - 1] + (target - y[k - 1]) * (t[k] - t[k - 1]) / (y[k] - y[k - 1]))
+
+
+def row_at(tqe, t):
+    return int(np.argmin(np.abs(tqe[:, 0] - t)))
+
+
+def compare(t64, p64, t32, p32, t_gate=2.0):
+    """Errors of the FP32-mixed series against the FP64 series (same input,
+    same cadence): at t = t_gate and maxima over the whole flow."""
+    k = row_at(t64, t_gate)
+    dE = np.abs(t32[:, 2] - t64[:, 2]) / np.abs(t64[:, 2])
+    dQ = np.abs(t32[:, 1] - t64[:, 1])
+    dP = np.abs(p32 - p64) / np.abs(p64)
+    return {
+        "t_gate": float(t64[k, 0]),
+        "dE_rel_at_t_gate": float(dE[k]),
+        "dQ_abs_at_t_gate": float(dQ[k]),
+        "dP_rel_at_t_gate": float(dP[k]),
+        "max_dE_rel": float(dE.max()),
+        "max_dQ_abs": float(dQ.max()),
+        "max_dP_rel": float(dP.max()),
+        "gate_dE_lt_1e-6": bool(dE[k] < 1e-6),
+        "gate_dQ_lt_1e-4": bool(dQ[k] < 1e-4),
+    }
+
+
+def stage_roofline(ms, launches, V, bytes_per_site_step):
+    stage_n = launches[0] + launches[1] + launches[2]
+    avg = (ms[0] + ms[1] + ms[2]) / max(1, stage_n)
+    gbs = V * bytes_per_site_step / 3.0 / (avg * 1e-3) / 1e9
+    return avg, gbs
+
+
+def setup_shape(qcbflow, h, args):
+    """launch shape: explicit knobs, then (unless disabled) the autotuner"""
+    for key, val in ((qcbflow.TUNE_MAP_MODE, args.map_mode),
+                     (qcbflow.TUNE_MAP_WIN, args.map_win),
+                     (qcbflow.TUNE_TILE_SHAPE, args.tile_shape)):
+        if val is not None:
+            h.tune(key, val)
+    if not args.no_autotune:
+        h.autotune(trial_steps=2)
+    return [h.get_tune(k) for k in (qcbflow.TUNE_MAP_MODE, qcbflow.TUNE_MAP_WIN,
+                                    qcbflow.TUNE_TILE_SHAPE)]
+
+
+def lattice_of(args, default):
+    if args.L is None:
+        return default
+    return tuple(args.L * 4) if len(args.L) == 1 else tuple(args.L)
+
+
+def volume(L):
+    V = 1
+    for v in L:
+        V *= v
+    return V
+
+
+def report_single(qcbflow, args, name, L, beta, seed, sweeps, K, every):
+    """FP64 and FP32-mixed flows of one thermalised lattice: throughput, stage
+    roofline, and the FP32-mixed errors against FP64 (P, E, Q, t0)."""
+    V = volume(L)
+    log(f"{name}: lattice {L}, V = {V}, beta {beta}, seed {seed}, {sweeps} heatbath sweeps")
+    t0 = time.perf_counter()
+    src = thermalised(qcbflow, L, beta, seed, sweeps)
+    log(f"  thermalised in {time.perf_counter() - t0:.2f} s, plaquette {src.plaquette():.6f}")
+    legs = {}
+    series = {}
+    for prec, tag, bps in ((qcbflow.FP64, "fp64", FLOW_BYTES_PER_SITE_STEP),
+                           (qcbflow.FP32_MIXED, "fp32_mixed", MIXED_BYTES_PER_SITE_STEP)):
+        h = qcbflow.GradientFlow(L, args.eps, prec)
+        h.copy_from(src)
+        shape = setup_shape(qcbflow, h, args)
+        secs, tqe, plaq, ms, n = timed_flow(h, src, K, args.warmup, every)
+        avg, gbs = stage_roofline(ms, n, V, bps)
+        legs[tag] = {
+            "value": V * K / secs,
+            "ms_per_rk3_step": secs / K * 1e3,
+            "stage_avg_ms": avg,
+            "hbm_achieved_gbs": gbs,
+            "hbm_frac": gbs / HBM_PEAK_GBS,
+            "bytes_per_site_step": bps,
+            "measure_avg_ms": ms[3] / max(1, n[3]),
+            "launch_shape": shape,
+            "exp_warnings": h.exp_warnings(),
+        }
+        series[tag] = (tqe, plaq)
+        log(f"  {tag}: {legs[tag]['value']:.4e} site-steps/s, stage {avg:.4f} ms "
+            f"({gbs:.0f} GB/s), E(t={tqe[-1, 0]:.2f}) = {tqe[-1, 2]:.6g}, Q = {tqe[-1, 1]:.6g}")
+        h.close()
+    src.close()
+    t64, p64 = series["fp64"]
+    t32, p32 = series["fp32_mixed"]
+    acc = compare(t64, p64, t32, p32)
+    # the reference's E is (1/2) sum |F|^2 / V with F = (C - C^dag)/8, half the
+    # usual normalisation, so t^2 E_ref = 0.15 is the customary t0 and 0.3 the
+    # reference's literal threshold
+    t0s = {}
+    for thr in (0.05, 0.1, 0.15, 0.3):
+        a, b = t_at(t64, thr), t_at(t32, thr)
+        t0s[f"{thr}"] = {"fp64": a, "fp32_mixed": b,
+                         "diff": None if a is None or b is None else abs(a - b)}
+    acc["t_at_t2E"] = t0s
+    log(f"  fp32-mixed vs fp64 at t={acc['t_gate']:.2f}: dE/E {acc['dE_rel_at_t_gate']:.2e}, "
+        f"dQ {acc['dQ_abs_at_t_gate']:.2e}, dP/P {acc['dP_rel_at_t_gate']:.2e}")
+    return {
+        "report": name,
+        "metric": METRIC,
+        "unit": UNIT,
+        "config": {"workload": f"Wilson flow, {'x'.join(map(str, L))}, eps={args.eps}, "
+                               f"{K} RK3 steps, measured every {every}",
+                   "lattice": list(L), "sites": V, "beta": beta, "seed": seed,
+                   "heatbath_sweeps": sweeps},
+        "fp64": legs["fp64"],
+        "fp32_mixed": legs["fp32_mixed"],
+        "fp32_mixed_vs_fp64": legs["fp32_mixed"]["value"] / legs["fp64"]["value"],
+        "accuracy_fp32_mixed_vs_fp64": acc,
+        "data": "synthetic: GPU Haar hot start + GPU heatbath",
+    }
+
+
+ENSEMBLE_SIZES = (8, 12, 16, 24)
+ENSEMBLE_BETAS = (5.8, 6.0, 6.2)
+
+
+def report_ensemble(qcbflow, args):
+    """64 configurations, 16 each of 8^4, 12^4, 16^4, 24^4, beta cycling over
+    {5.8, 6.0, 6.2}, Haar hot starts from seeds 100..163 thermalised by 30
+    heatbath sweeps, flowed 150 RK3 steps together (persistent stage kernels,
+    graph-replayed steps, autotuned launch shape) in FP32-mixed and FP64,
+    against the same configurations flowed one at a time."""
+    K = args.steps if args.steps_given else 150
+    sweeps = args.sweeps if args.sweeps is not None else 30
+    dims = [(ENSEMBLE_SIZES[c // 16],) * 4 for c in range(64)]
+    betas = [ENSEMBLE_BETAS[c % 3] for c in range(64)]
+    seeds = [100 + c for c in range(64)]
+    total_v = sum(volume(L) for L in dims)
+    log(f"ensemble: 64 configurations, sum of volumes {total_v}, {K} steps, {sweeps} sweeps")
+    ens = {p: qcbflow.Ensemble(dims, args.eps, p) for p in (qcbflow.FP64, qcbflow.FP32_MIXED)}
+    seq = {qcbflow.FP64: [], qcbflow.FP32_MIXED: []}
+    seq_secs = {qcbflow.FP64: 0.0, qcbflow.FP32_MIXED: 0.0}
+    singles = {}
+    t_start = time.perf_counter()
+    for c in range(64):
+        L = dims[c]
+        src = thermalised(qcbflow, L, betas[c], seeds[c], sweeps)
+        for p in ens:
+            ens[p].copy_from(c, src)
+            if (L, p) not in singles:
+                singles[(L, p)] = qcbflow.GradientFlow(L, args.eps, p)
+                singles[(L, p)].copy_from(src)
+                singles[(L, p)].flow_measure(3, 1)  # code objects loaded before timing
+            h = singles[(L, p)]
+            h.copy_from(src)
+            h.synchronize()
+            t0 = time.perf_counter()
+            tqe, plaq = h.flow_measure(K + 1, 1)
+            h.synchronize()
+            seq_secs[p] += time.perf_counter() - t0
+            seq[p].append((tqe, plaq))
+        src.close()
+    log(f"  thermalised + sequential flows in {time.perf_counter() - t_start:.1f} s")
+    for h in singles.values():
+        h.close()
+    out = {"report": "ensemble", "metric": METRIC, "unit": UNIT,
+           "config": {"workload": "ensemble of 64 lattices (16 each of 8^4, 12^4, 16^4, 24^4), "
+                                  f"eps={args.eps}, {K} RK3 steps, measured every step",
+                      "sizes": list(ENSEMBLE_SIZES), "betas": list(ENSEMBLE_BETAS),
+                      "seeds": [seeds[0], seeds[-1]], "heatbath_sweeps": sweeps,
+                      "sum_sites": total_v},
+           "data": "synthetic: GPU Haar hot starts + GPU heatbath"}
+    res = {}
+    for p, tag in ((qcbflow.FP64, "fp64"), (qcbflow.FP32_MIXED, "fp32_mixed")):
+        e = ens[p]
+        e.use_graph(True)
+        m, w, b = (None, None, None)
+        if not args.no_autotune:
+            m, w, b = e.autotune(trial_steps=2)
+        e.synchronize()
+        t0 = time.perf_counter()
+        tqe, plaq = e.flow_measure(K + 1, 1)
+        e.synchronize()
+        secs = time.perf_counter() - t0
+        same = all(np.array_equal(tqe[c], seq[p][c][0]) and np.array_equal(plaq[c], seq[p][c][1])
+                   for c in range(64))
+        res[tag] = (tqe, plaq)
+        out[tag] = {
+            "value": total_v * K / secs,
+            "seconds": secs,
+            "sequential_value": total_v * K / seq_secs[p],
+            "speedup_vs_sequential": seq_secs[p] / secs,
+            "launch_shape": {"map_mode": m, "map_win": w, "blocks_per_cu": b},
+            "bit_identical_to_single_lattice_path": bool(same),
+            "exp_warnings": e.exp_warnings(),
+        }
+        log(f"  {tag}: ensemble {out[tag]['value']:.4e} site-steps/s, sequential "
+            f"{out[tag]['sequential_value']:.4e} ({out[tag]['speedup_vs_sequential']:.2f}x), "
+            f"identical to single-lattice path: {same}")
+        e.close()
+    out["fp32_mixed_vs_fp64"] = out["fp32_mixed"]["value"] / out["fp64"]["value"]
+    per = [compare(res["fp64"][0][c], res["fp64"][1][c], res["fp32_mixed"][0][c],
+                   res["fp32_mixed"][1][c]) for c in range(64)]
+    out["accuracy_fp32_mixed_vs_fp64"] = {
+        k: max(d[k] for d in per) for k in ("dE_rel_at_t_gate", "dQ_abs_at_t_gate",
+                                            "dP_rel_at_t_gate", "max_dE_rel", "max_dQ_abs",
+                                            "max_dP_rel")}
+    out["accuracy_fp32_mixed_vs_fp64"]["t_gate"] = per[0]["t_gate"]
+    out["accuracy_fp32_mixed_vs_fp64"]["note"] = "maximum over the 64 configurations"
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed RK3 steps (default 200; 250/150/400 for the reports)")
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--L", type=int, nargs="+", default=[32])
+    ap.add_argument("--L", type=int, nargs="+", default=None)
     ap.add_argument("--eps", type=float, default=0.02)
     ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--beta", type=float, default=6.0)
+    ap.add_argument("--sweeps", type=int, default=None,
+                    help="heatbath sweeps of the input (default 50)")
+    ap.add_argument("--report", choices=("precision", "ensemble", "large"), default=None)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
@@ -119,49 +367,56 @@ def main():
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run the plain copy kernel on this many MiB (traffic calibration)")
     args = ap.parse_args()
+    args.steps_given = args.steps is not None
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
               "the whole lattice lives on a single MI355X.", file=sys.stderr)
         sys.exit(2)
 
-    L = tuple(args.L * 4) if len(args.L) == 1 else tuple(args.L)
-    V = 1
-    for v in L:
-        V *= v
     import qcbflow
 
     name, cus = qcbflow.device_info()
-    log(f"device {name}, {cus} CUs; lattice {L}, V = {V}")
-    def setup(h):
-        """launch shape: explicit knobs, then (unless disabled) the autotuner"""
-        for key, val in ((qcbflow.TUNE_MAP_MODE, args.map_mode),
-                         (qcbflow.TUNE_MAP_WIN, args.map_win),
-                         (qcbflow.TUNE_TILE_SHAPE, args.tile_shape)):
-            if val is not None:
-                h.tune(key, val)
-        if not args.no_autotune:
-            h.autotune(trial_steps=2)
-        return [h.get_tune(k) for k in (qcbflow.TUNE_MAP_MODE, qcbflow.TUNE_MAP_WIN,
-                                        qcbflow.TUNE_TILE_SHAPE)]
+    log(f"device {name}, {cus} CUs")
+    if args.report == "precision":
+        K = args.steps if args.steps_given else 250
+        out = report_single(qcbflow, args, "precision", lattice_of(args, (48,) * 4), 6.2, 11,
+                            args.sweeps if args.sweeps is not None else 100, K, 1)
+        print(json.dumps(out), flush=True)
+        return
+    if args.report == "large":
+        K = args.steps if args.steps_given else 400
+        out = report_single(qcbflow, args, "large", lattice_of(args, (64,) * 4), 6.4, 2026,
+                            args.sweeps if args.sweeps is not None else 50, K, 10)
+        print(json.dumps(out), flush=True)
+        return
+    if args.report == "ensemble":
+        print(json.dumps(report_ensemble(qcbflow, args)), flush=True)
+        return
+
+    L = lattice_of(args, (32,) * 4)
+    V = volume(L)
+    K = args.steps if args.steps_given else 200
+    sweeps = args.sweeps if args.sweeps is not None else 50
+    log(f"lattice {L}, V = {V}; hot start seed {args.seed}, {sweeps} heatbath sweeps at "
+        f"beta {args.beta}")
+    t0 = time.perf_counter()
+    src = thermalised(qcbflow, L, args.beta, args.seed, sweeps)
+    log(f"thermalised in {time.perf_counter() - t0:.2f} s, plaquette {src.plaquette():.6f}")
 
     gf = qcbflow.GradientFlow(L, args.eps)
-    gf.hot_start(args.seed)
-    shape64 = setup(gf)
+    gf.copy_from(src)
+    shape64 = setup_shape(qcbflow, gf, args)
     log(f"fp64 launch shape (map mode, window, tile code): {shape64}")
 
     # timed region: K RK3 steps, K+1 measurements, stage kernels bracketed by events
-    K = args.steps
-    secs, tqe, plaq, ms, launches = timed_flow(gf, K, args.warmup)
+    secs, tqe, plaq, ms, launches = timed_flow(gf, src, K, args.warmup)
     value = V * K / secs
     log(f"timed: {K} steps in {secs:.4f} s -> {value:.4e} site-steps/s; "
         f"E(t={tqe[-1, 0]:.2f}) = {tqe[-1, 2]:.6g}, Q = {tqe[-1, 1]:.6g}")
 
-    stage_ms = ms[0] + ms[1] + ms[2]
-    stage_n = launches[0] + launches[1] + launches[2]
-    avg_stage_ms = stage_ms / stage_n
+    avg_stage_ms, achieved = stage_roofline(ms, launches, V, FLOW_BYTES_PER_SITE_STEP)
     bytes_per_launch = V * FLOW_BYTES_PER_SITE_STEP / 3.0
-    achieved = bytes_per_launch / (avg_stage_ms * 1e-3) / 1e9
     per_stage = {f"stage{k + 1}_avg_ms": ms[k] / max(1, launches[k]) for k in range(3)}
     meas_avg_ms = ms[3] / max(1, launches[3])
     flops_rate = V * FLOW_FLOPS_PER_SITE_STEP / (3 * avg_stage_ms * 1e-3) / 1e12
@@ -179,17 +434,16 @@ def main():
         copy_ms = gf.copy_bench(args.calibrate_copy_mib << 20, 10)
         log(f"copy kernel: {args.calibrate_copy_mib} MiB in {copy_ms:.4f} ms -> "
             f"{2 * (args.calibrate_copy_mib << 20) / (copy_ms * 1e-3) / 1e9:.1f} GB/s (read+write)")
+    gf.close()
 
-    # the FP32-mixed form on the same lattice and cadence
+    # the FP32-mixed form on the same input and cadence
     mixed = None
     if not args.no_mixed:
-        links = gf.field() if not args.no_cpu else None
         gm = qcbflow.GradientFlow(L, args.eps, qcbflow.FP32_MIXED)
-        gm.hot_start(args.seed)
-        shape32 = setup(gm)
-        msecs, mtqe, mplaq, mms, mn = timed_flow(gm, K, args.warmup)
-        mstage = (mms[0] + mms[1] + mms[2]) / (mn[0] + mn[1] + mn[2])
-        mach = V * MIXED_BYTES_PER_SITE_STEP / 3.0 / (mstage * 1e-3) / 1e9
+        gm.copy_from(src)
+        shape32 = setup_shape(qcbflow, gm, args)
+        msecs, mtqe, mplaq, mms, mn = timed_flow(gm, src, K, args.warmup)
+        mstage, mach = stage_roofline(mms, mn, V, MIXED_BYTES_PER_SITE_STEP)
         mixed = {
             "value": V * K / msecs,
             "vs_fp64": (V * K / msecs) / value,
@@ -199,28 +453,22 @@ def main():
             "hbm_achieved_gbs": mach,
             "hbm_frac": mach / HBM_PEAK_GBS,
             "bytes_per_site_step": MIXED_BYTES_PER_SITE_STEP,
-            "final_t": float(mtqe[-1, 0]),
-            "dE_rel_vs_fp64_at_final_t": float(abs(mtqe[-1, 2] - tqe[-1, 2]) / tqe[-1, 2]),
-            "dQ_vs_fp64_at_final_t": float(abs(mtqe[-1, 1] - tqe[-1, 1])),
+            "fp32_tflops": V * MIXED_FLOPS_PER_SITE_STEP / (3 * mstage * 1e-3) / 1e12,
             "launch_shape": shape32,
-            "note": "same Haar hot start rounded to FP32; accuracy gates on thermalised "
-                    "lattices are in tests/test_gpu_mixed.py",
+            "accuracy_vs_fp64": compare(tqe, plaq, mtqe, mplaq),
         }
         log(f"fp32-mixed: {mixed['value']:.4e} site-steps/s ({mixed['vs_fp64']:.2f}x fp64), "
-            f"stage {mstage:.4f} ms")
+            f"stage {mstage:.4f} ms, dE/E at t=2 {mixed['accuracy_vs_fp64']['dE_rel_at_t_gate']:.2e}")
         gm.close()
-    else:
-        links = None
 
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
-        if links is None:
-            links = gf.field()
+        links = src.field()
         threads = min(args.cpu_threads, os.cpu_count() or 1)
         log(f"cpu baseline: {args.cpu_steps} steps on {threads} threads ...")
         cpu = cpu_baseline(links, L, args.eps, args.cpu_steps, threads)
         log(f"cpu baseline: {cpu['value']:.4e} {UNIT}")
-    gf.close()
+    src.close()
 
     out = {
         "metric": METRIC,
@@ -234,14 +482,16 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": f"synthetic: Haar-random SU(3) hot start (seed {args.seed}) generated on the GPU; "
-                "the flow's cost is data-independent (fixed 18-step Horner, no branches)",
+        "data": f"synthetic: Haar-random SU(3) hot start (seed {args.seed}) thermalised by "
+                f"{sweeps} heatbath sweeps at beta={args.beta}, generated on the GPU",
         "config": {
             "workload": f"Wilson flow FP64, {'x'.join(map(str, L))} lattice, eps={args.eps}, "
                         "plaquette+Q+E measured every RK3 step",
             "lattice": list(L),
             "sites": V,
             "eps": args.eps,
+            "beta": args.beta,
+            "heatbath_sweeps": sweeps,
             "measure_every_rk3_steps": 1,
             "precision": "fp64",
             "parallelism": "single GPU",
@@ -249,7 +499,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_rk3_stage64<1,2,3> (fused RK3 stage)",
+            "kernel": "k_rk3_stage<f64,1|2|3> (fused RK3 stage)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

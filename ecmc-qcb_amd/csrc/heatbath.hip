@@ -4,8 +4,8 @@
 // Restates heatbath::generate_su2_kp (src/heatbath/heatbath.cpp:8-36),
 // su2_step (:39-44) and mpi::heatbathcb::hit (src/heatbath/heatbath_mpi.cpp:
 // 10-33): for each link, the staple A, then for the SU(2) subgroups (0,1),
-// (1,2), (0,2): W = U A, w = the quaternion of W's 2x2 block, a Kennedy-
-// Pendleton draw x at alpha = 2 beta |w| / 3, U <- embed(x (w/|w|)^dag) U.
+// (1,2), (0,2): W = U A, w = the quaternion of W's 2x2 block, an SU(2) draw
+// x at alpha = 2 beta |w| / 3 (su2_draw), U <- embed(x (w/|w|)^dag) U.
 // The sweep order differs from the reference on purpose: one launch updates
 // all U_mu(x) of one site parity for one mu, and such links share no staple
 // on a lattice with even extents, so the update is race-free (the
@@ -32,21 +32,43 @@ struct hb_rng {
     }
 };
 
-// Kennedy-Pendleton sample of a0 with density ~ sqrt(1 - a0^2) exp(alpha a0),
-// then a uniform direction; kMaxTries bounds the accept loop (the acceptance
-// is above 0.7 for any alpha, so the bound is never the normal exit).
+// Draw a in SU(2) (as a quaternion) with density ~ exp(alpha a0) on S^3:
+// a0 with density ~ sqrt(1 - a0^2) exp(alpha a0), then a uniform direction.
+// For alpha >= kCreutzBelow, Kennedy-Pendleton (heatbath.cpp:8-36): lambda^2
+// from two exponentials and a cosine, accepted when delta^2 <= 1 - lambda^2.
+// Its acceptance falls to zero as alpha -> 0 (0.28 at alpha = 0.5, 1e-3 at
+// 0.01), and a hot-started lattice does meet such links, so below the
+// threshold a0 comes from Creutz's method instead (M. Creutz, Phys. Rev. D 21
+// (1980) 2308): a0 from exp(alpha a0) on [-1, 1] by inverting its CDF,
+// accepted with probability sqrt(1 - a0^2).  Both target the same density;
+// the acceptance of the pair is above 0.68 for every alpha > 0, so kMaxTries
+// bounds the loop without ever being the normal exit.
 constexpr int kMaxTries = 64;
-__device__ inline bool su2_kp(double alpha, hb_rng& r, double a[4]) {
+constexpr double kCreutzBelow = 2.0;
+__device__ inline bool su2_draw(double alpha, hb_rng& r, double a[4]) {
     double a0 = 0.0;
     bool ok = false;
-    for (int k = 0; k < kMaxTries; k++) {
-        const double delta = r.unif(), r1 = r.unif(), r2 = r.unif(), r3 = r.unif();
-        const double cs = cos(2.0 * M_PI * r2);
-        const double lsq = -0.5 / alpha * (log(r1) + cs * cs * log(r3));
-        if (delta * delta <= 1.0 - lsq) {
-            a0 = 1.0 - 2.0 * lsq;
-            ok = true;
-            break;
+    if (alpha >= kCreutzBelow) {
+        for (int k = 0; k < kMaxTries; k++) {
+            const double delta = r.unif(), r1 = r.unif(), r2 = r.unif(), r3 = r.unif();
+            const double cs = cos(2.0 * M_PI * r2);
+            const double lsq = -0.5 / alpha * (log(r1) + cs * cs * log(r3));
+            if (delta * delta <= 1.0 - lsq) {
+                a0 = 1.0 - 2.0 * lsq;
+                ok = true;
+                break;
+            }
+        }
+    } else {
+        const double span = expm1(-2.0 * alpha);  // e^{-2 alpha} - 1, in (-1, 0)
+        for (int k = 0; k < kMaxTries; k++) {
+            const double u = r.unif(), delta = r.unif();
+            const double x = fmin(1.0, fmax(-1.0, 1.0 + log1p((1.0 - u) * span) / alpha));
+            if (delta * delta <= 1.0 - x * x) {
+                a0 = x;
+                ok = true;
+                break;
+            }
         }
     }
     if (!ok) return false;
@@ -78,7 +100,7 @@ __device__ inline void hb_link(double2* __restrict__ U, const lattice& g, unsign
                              (w00.i - w11.i) / 2.0};
         const double k = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2] + w[3] * w[3]);
         double x[4];
-        if (!(k > 0.0) || !su2_kp(beta * k * 2.0 / 3.0, r, x)) {
+        if (!(k > 0.0) || !su2_draw(beta * k * 2.0 / 3.0, r, x)) {
             stuck = 1;
             continue;
         }

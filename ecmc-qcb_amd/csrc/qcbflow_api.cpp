@@ -15,6 +15,7 @@
 #include <string>
 #include <vector>
 
+#include "api_internal.hpp"
 #include "ildg.hpp"
 #include "kernels.hpp"
 #include "qcbflow.h"
@@ -256,6 +257,38 @@ extern "C" int qcbf_get_links(qcbf_flow* h, double* links) {
     return QCBF_OK;
 }
 
+extern "C" int qcbf_copy_field(qcbf_flow* dst, qcbf_flow* src) {
+    CHECK_H(dst);
+    if (!src) return fail(dst, QCBF_ERR_ARG, "null source handle");
+    if (src->dev != dst->dev) return fail(dst, QCBF_ERR_ARG, "handles on different devices");
+    for (int d = 0; d < 4; d++)
+        if (src->g.L[d] != dst->g.L[d]) return fail(dst, QCBF_ERR_ARG, "extents differ");
+    if (src == dst) return QCBF_OK;
+    int rc = ensure_stage(dst);
+    if (rc) return rc;
+    // the source's queued work must be done before its field is read
+    HIPCHK(dst, hipStreamSynchronize(src->stream));
+    double* st = static_cast<double*>(dst->stage);
+    for (unsigned s0 = 0; s0 < dst->g.V; s0 += kChunkSites) {
+        const unsigned n = std::min(kChunkSites, dst->g.V - s0);
+        HIPCHK(dst, qcb::launch_field_to_aos(src->prec, src->U[src->cur], st, src->g, s0, n, false,
+                                             dst->stream));
+        HIPCHK(dst, qcb::launch_aos_to_field(dst->prec, st, 0, dst->U[dst->cur], dst->g, s0, n,
+                                             dst->stream));
+    }
+    HIPCHK(dst, hipStreamSynchronize(dst->stream));
+    return QCBF_OK;
+}
+
+// The ensemble ABI copies from a single-lattice handle through this view.
+int qcb::field_view(qcbf_flow* h, int* prec, const void** field, hipStream_t* stream) {
+    if (!h) return QCBF_ERR_ARG;
+    *prec = (int)h->prec;
+    *field = h->U[h->cur];
+    *stream = h->stream;
+    return QCBF_OK;
+}
+
 extern "C" int qcbf_hot_start(qcbf_flow* h, uint64_t seed) {
     CHECK_H(h);
     HIPCHK(h, qcb::launch_hot_start(h->prec, h->U[h->cur], h->g, seed, h->stream));
@@ -279,7 +312,7 @@ extern "C" int qcbf_heatbath(qcbf_flow* h, double beta, int n_sweeps, uint64_t s
     HIPCHK(h, hipMemcpyAsync(&stuck, h->dwarn, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipMemsetAsync(h->dwarn, 0, sizeof(int), h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    if (stuck) return fail(h, QCBF_ERR_STATE, "a Kennedy-Pendleton draw hit its retry bound");
+    if (stuck) return fail(h, QCBF_ERR_STATE, "an SU(2) heatbath draw hit its retry bound");
     return QCBF_OK;
 }
 

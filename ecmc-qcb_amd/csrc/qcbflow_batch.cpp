@@ -16,6 +16,7 @@
 #include <string>
 #include <vector>
 
+#include "api_internal.hpp"
 #include "batch.hpp"
 #include "kernels.hpp"
 #include "qcbflow.h"
@@ -297,6 +298,32 @@ extern "C" int qcbf_batch_hot_start(qcbf_batch* b, int c, uint64_t seed) {
     int rc = check_cfg(b, c);
     if (rc) return rc;
     BCHK(b, qcb::launch_hot_start(b->prec, field_of(b, c), b->cfgs[c].g, seed, b->stream));
+    return QCBF_OK;
+}
+
+extern "C" int qcbf_batch_copy_from(qcbf_batch* b, int c, qcbf_flow* src) {
+    CHECK_B(b);
+    int rc = check_cfg(b, c);
+    if (rc) return rc;
+    int sp = 0;
+    const void* sf = nullptr;
+    hipStream_t ss = nullptr;
+    if (!src || qcb::field_view(src, &sp, &sf, &ss) != QCBF_OK)
+        return bfail(b, QCBF_ERR_ARG, "bad source handle");
+    int sdims[4], sprec = 0;
+    qcbf_get_dims(src, sdims, &sprec);
+    const qcb::lattice& g = b->cfgs[c].g;
+    for (int d = 0; d < 4; d++)
+        if (sdims[d] != g.L[d]) return bfail(b, QCBF_ERR_ARG, "extents differ");
+    if ((rc = ensure_stage(b))) return rc;
+    BCHK(b, hipStreamSynchronize(ss));
+    double* st = static_cast<double*>(b->stage);
+    for (unsigned s0 = 0; s0 < g.V; s0 += kChunkSites) {
+        const unsigned n = std::min(kChunkSites, g.V - s0);
+        BCHK(b, qcb::launch_field_to_aos((qcb::prec)sp, sf, st, g, s0, n, false, b->stream));
+        BCHK(b, qcb::launch_aos_to_field(b->prec, st, 0, field_of(b, c), g, s0, n, b->stream));
+    }
+    BCHK(b, hipStreamSynchronize(b->stream));
     return QCBF_OK;
 }
 

@@ -55,6 +55,8 @@ SIGNATURES = {
     "qcbf_set_links": (ctypes.c_int, [_vp, _dp]),
     "qcbf_get_links": (ctypes.c_int, [_vp, _dp]),
     "qcbf_hot_start": (ctypes.c_int, [_vp, ctypes.c_uint64]),
+    "qcbf_copy_field": (ctypes.c_int, [_vp, _vp]),
+    "qcbf_batch_copy_from": (ctypes.c_int, [_vp, ctypes.c_int, _vp]),
     "qcbf_cold_start": (ctypes.c_int, [_vp]),
     "qcbf_heatbath": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_uint64]),
     "qcbf_load_ildg": (ctypes.c_int, [_vp, ctypes.c_char_p]),
@@ -220,6 +222,10 @@ class GradientFlow:
     def hot_start(self, seed):
         self._c(load().qcbf_hot_start(self._h, int(seed)))
 
+    def copy_from(self, other):
+        """field_c <- other's field, on the device (precisions may differ)."""
+        self._c(load().qcbf_copy_field(self._h, other._h))
+
     def cold_start(self):
         self._c(load().qcbf_cold_start(self._h))
 
@@ -376,6 +382,10 @@ class Ensemble:
 
     def hot_start(self, c, seed):
         self._c(load().qcbf_batch_hot_start(self._h, int(c), int(seed)))
+
+    def copy_from(self, c, flow):
+        """configuration c <- the field of a GradientFlow handle, on the device."""
+        self._c(load().qcbf_batch_copy_from(self._h, int(c), flow._h))
 
     def rk3_step(self, n=1, eps=None):
         self._c(load().qcbf_batch_rk3_step(self._h, self.epsilon if eps is None else float(eps),

@@ -88,6 +88,12 @@ int qcbf_set_links(qcbf_flow* h, const double* links);
 /* Read the flowing field back (GradientFlow::field_c). */
 int qcbf_get_links(qcbf_flow* h, double* links);
 
+/* Copy the flowing field of src into dst on the device (same extents; the
+ * precisions may differ: FP64 -> FP32-mixed re-unitarises and rounds, the
+ * other way rebuilds the third row in FP64).  Both handles must be on the
+ * same device. */
+int qcbf_copy_field(qcbf_flow* dst, qcbf_flow* src);
+
 /* GaugeField::hot_start / cold_start (src/gauge/GaugeField.cpp:10-33):
  * Haar-random SU(3) links from a counter-based generator keyed by seed, or the
  * identity.  Generated on the device. */
@@ -203,6 +209,8 @@ int qcbf_batch_size(const qcbf_batch* b, int* n, int* dims);
 int qcbf_batch_set_links(qcbf_batch* b, int c, const double* links);
 int qcbf_batch_get_links(qcbf_batch* b, int c, double* links);
 int qcbf_batch_hot_start(qcbf_batch* b, int c, uint64_t seed);
+/* Configuration c <- the flowing field of src (same extents), on the device. */
+int qcbf_batch_copy_from(qcbf_batch* b, int c, qcbf_flow* src);
 int qcbf_batch_rk3_step(qcbf_batch* b, double eps, int n_steps);
 /* {P, E, Q} of every configuration: out[3c..3c+2] */
 int qcbf_batch_measure(qcbf_batch* b, double* out);

@@ -124,6 +124,23 @@ def test_single_lattice_autotune_and_tile_maps_agree():
         assert np.array_equal(r[0], results[0][0]) and np.array_equal(r[1], results[0][1])
 
 
+@pytest.mark.parametrize("prec", [qcbflow.FP64, qcbflow.FP32_MIXED])
+def test_batch_copy_from_single_lattice_handles(prec):
+    """Ensemble.copy_from (device to device, FP64 source) equals a host copy."""
+    dims = [(4, 4, 4, 4), (6, 4, 8, 4)]
+    fs = fields_for(dims, sweeps=1)
+    with qcbflow.Ensemble(dims, 0.02, prec) as dev, qcbflow.Ensemble(dims, 0.02, prec) as host:
+        for c, (L, f) in enumerate(zip(dims, fs)):
+            with qcbflow.GradientFlow(L) as gf:
+                gf.copy(f)
+                dev.copy_from(c, gf)
+            host.copy(c, f)
+            assert np.array_equal(dev.field(c), host.field(c)), c
+        with qcbflow.GradientFlow(dims[0]) as gf:
+            with pytest.raises(qcbflow.QcbfError):
+                dev.copy_from(1, gf)
+
+
 def test_batch_errors():
     with pytest.raises(qcbflow.QcbfError) as ei:
         qcbflow.Ensemble([(4, 4, 4, 4), (4, 1, 4, 4)])

@@ -110,3 +110,28 @@ def test_mixed_ildg_load(golden_dir):
         gf.load_ildg(path)
         got = gf.field()
     assert np.abs(got - want).max() < 2e-7
+
+
+def test_device_copies_between_handles_match_host_copies():
+    """qcbf_copy_field converts on the device through the same kernels as a
+    host round trip: FP64 -> FP64 is exact, FP64 -> FP32-mixed and back give
+    what copying the host field gives."""
+    L = (8, 6, 4, 10)
+    f = thermal(L, 21, 2)
+    with qcbflow.GradientFlow(L) as a, qcbflow.GradientFlow(L) as b, \
+            qcbflow.GradientFlow(L, precision=qcbflow.FP32_MIXED) as m, \
+            qcbflow.GradientFlow(L, precision=qcbflow.FP32_MIXED) as m_host, \
+            qcbflow.GradientFlow(L) as back_host:
+        a.copy(f)
+        b.copy_from(a)
+        assert np.array_equal(b.field(), f)
+        m.copy_from(a)
+        m_host.copy(f)
+        assert np.array_equal(m.field(), m_host.field())
+        b.copy_from(m)
+        back_host.copy(m.field())
+        assert np.array_equal(b.field(), back_host.field())
+        assert np.array_equal(m.flow_measure(3, 1)[0], m_host.flow_measure(3, 1)[0])
+        with qcbflow.GradientFlow((8, 6, 4, 8)) as other:
+            with pytest.raises(qcbflow.QcbfError):
+                other.copy_from(a)
