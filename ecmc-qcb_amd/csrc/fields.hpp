@@ -12,12 +12,20 @@
 //          reference's proj_su3, src/su3/utils.cpp:194-214, does the same by
 //          columns on full matrices; with two stored rows the row form is the
 //          natural one), so the stored field stays in SU(3) to FP32 rounding.
+// The FP64 flow output may hold rows 0-1 only (store_flow_link64,
+// lattice.hpp); the FP32-mixed form never stores the third row.
 // su(3) fields (force / RK3 accumulator) hold the 8 reals of alg8:
 //          FP64 as 4 double2 per link, FP32 as 2 float4 per link.
 #pragma once
 #include <hip/hip_runtime.h>
 
 #include "lattice.hpp"
+
+// Products of links (staples, clover leaves) formed as SU(3) products: two
+// rows and a cross product (mul_su3 and friends, su3.hpp).  0 = full products.
+#ifndef QCB_SU3_PRODUCTS
+#define QCB_SU3_PRODUCTS 1
+#endif
 
 namespace qcb {
 
@@ -122,8 +130,9 @@ struct storage<prec::f64> {
     static constexpr int link_words = 9;
     static constexpr int alg_words = 4;
     __device__ static field view(const vec* p, unsigned V) { return field{p, V}; }
+    // output of a flow update (two rows when the readers rebuild the third)
     __device__ static void store(vec* p, unsigned V, int mu, unsigned s, const mat3<double>& m) {
-        store_link64(p, V, mu, s, m);
+        store_flow_link64(p, V, mu, s, m);
     }
     __device__ static alg8<double> load_alg(const avec* p, unsigned V, int mu, unsigned s) {
         return load_alg64(p, V, mu, s);

@@ -37,7 +37,8 @@ __global__ __launch_bounds__(256, QCB_STAGE_WAVES_PER_EU) void k_rk3_stage(
     typename storage<P>::avec* __restrict__ acc, lattice g, double cz, double ca,
     int* __restrict__ warn) {
     const unsigned ntiles = stage_tiles(g);
-    const unsigned tile = tile_of_block(blockIdx.x, ntiles, g.map_mode, g.map_win);
+    const unsigned tile =
+        tile_of_block(blockIdx.x, ntiles, g.map_mode, g.map_win, stage_tiles_per_t(g));
     if (tile >= ntiles) return;
     unsigned c[4];
     const unsigned s = tile_site(g, tile, threadIdx.x, c);
@@ -62,7 +63,7 @@ static hipError_t stage_launch(int stage, const void* Uin, void* Uout, void* acc
     auto* out = static_cast<typename S::vec*>(Uout);
     auto* a = static_cast<typename S::avec*>(acc);
     const dim3 block(kSitesPerBlock, 4);
-    const dim3 grid(grid_for_tiles(stage_tiles(g), g.map_mode, g.map_win));
+    const dim3 grid(grid_for_tiles(stage_tiles(g), g.map_mode, g.map_win, stage_tiles_per_t(g)));
     switch (stage) {
         case 1: k_rk3_stage<P, 1><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
         case 2: k_rk3_stage<P, 2><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;

@@ -73,6 +73,13 @@ __device__ inline mat3<typename F::real> staple_sum(const F& f, unsigned s, cons
                         for (int j = 0; j < 3; j++) st.e[i][j] = madd_cl(st.e[i][j], P.e[k][i], v[j]);
                 }
             }
+        } else if constexpr (QCB_SU3_PRODUCTS) {
+            // every product of links is in SU(3): form two rows, rebuild the third
+            add_to(st, mul_nd_su3(f.link(nu, xmu), mul_su3(f.link(nu, s), f.link(MU, xnu))));
+#if QCB_SCHED_SPLIT >= 2
+            if constexpr (sizeof(T) == 8) __builtin_amdgcn_sched_barrier(0);
+#endif
+            add_to(st, mul_dn_su3(mul_su3(f.link(MU, xmnu), f.link(nu, xmunu)), f.link(nu, xmnu)));
         } else {
             {
                 const mat3<T> a = mul(f.link(nu, s), f.link(MU, xnu));

@@ -25,8 +25,8 @@ struct lattice {
     unsigned V;         // number of sites
     unsigned stride[4]; // 1, Lx, Lx*Ly, Lx*Ly*Lz
     // workgroup -> site-tile order of the flow kernels (see tile_of_block)
-    int map_mode;       // 0 linear, 1 one contiguous chunk per XCD, 2 XCD chunks per window
-    unsigned map_win;   // window in workgroups for map_mode 2 (multiple of 8)
+    int map_mode;       // 0 linear, 1 one chunk per XCD, 2 XCD chunks per window, 3 slab sweep
+    unsigned map_win;   // window (mode 2, workgroups) or slab width (mode 3, tiles)
     // Shape of the 64-site tile of one stage workgroup: lg_tile[d] = log2 of its
     // extent along d (sum 6), or lg_tile[0] < 0 for 64 consecutive sites.  A
     // box tile needs every L[d] divisible by its extent.
@@ -60,29 +60,69 @@ __host__ __device__ inline unsigned tile_site(const lattice& g, unsigned tile, u
     return ((c[3] * (unsigned)g.L[2] + c[2]) * (unsigned)g.L[1] + c[1]) * (unsigned)g.L[0] + c[0];
 }
 
+// Tiles of one t-slice for the slab order (0 when 64-site tiles straddle
+// t-slices): stage tiles (64 consecutive sites or a box) and the measurement's
+// 64 consecutive sites.
+__host__ __device__ inline unsigned stage_tiles_per_t(const lattice& g) {
+    if (g.lg_tile[0] >= 0) return g.ntile[0] * g.ntile[1] * g.ntile[2];
+    const unsigned n = (unsigned)g.L[0] * (unsigned)g.L[1] * (unsigned)g.L[2];
+    return n % 64u == 0u ? n / 64u : 0u;
+}
+__host__ __device__ inline unsigned site_tiles_per_t(const lattice& g) {
+    const unsigned n = (unsigned)g.L[0] * (unsigned)g.L[1] * (unsigned)g.L[2];
+    return n % 64u == 0u ? n / 64u : 0u;
+}
+
 // The dispatcher hands workgroups to the 8 XCDs round-robin (workgroup b runs
 // on XCD b % 8), and each XCD has its own L2.  These orders give every XCD a
 // contiguous run of site tiles so that neighbouring tiles share its L2:
 //   mode 1: XCD k takes tiles [k q, (k+1) q), q = ceil(ntiles / 8);
-//   mode 2: the same inside consecutive windows of map_win workgroups, so all
-//           XCDs still sweep the lattice together (t-neighbours stay in the
-//           shared last-level cache).
+//   mode 2: the same inside consecutive windows of win workgroups (win rounded
+//           down to a multiple of 8), so all XCDs still sweep the lattice
+//           together (t-neighbours stay in the shared last-level cache);
+//   mode 3: slab sweep.  The per_t tiles of one t-slice are cut into slabs of
+//           win tiles (slab = a run of z-slices when win is a multiple of the
+//           tiles of one z-slice); XCD k owns slabs k, k+8, ... and walks each
+//           through t, t-slice by t-slice.  The tiles at t-1, t and t+1 of a
+//           slab are then at most 2 win dispatches apart on the same XCD, so
+//           the t-neighbours of a link are still in that XCD's L2 when they
+//           are read again.  Needs per_t > 0 (tiles never straddle t-slices);
+//           otherwise mode 3 is the linear order.
 // Returns a tile index >= ntiles for padding workgroups, which exit.
 __host__ __device__ inline unsigned tile_of_block(unsigned b, unsigned ntiles, int mode,
-                                                  unsigned win) {
+                                                  unsigned win, unsigned per_t = 0u) {
     if (mode == 1) {
         const unsigned q = (ntiles + 7) / 8;
         return (b % 8) * q + b / 8;
     }
     if (mode == 2) {
-        const unsigned w = b / win, j = b % win;
-        return w * win + (j % 8) * (win / 8) + j / 8;
+        const unsigned w8 = win < 8u ? 8u : (win & ~7u);
+        const unsigned w = b / w8, j = b % w8;
+        return w * w8 + (j % 8) * (w8 / 8) + j / 8;
+    }
+    if (mode == 3 && per_t > 0u) {
+        const unsigned R = win < 1u ? 1u : win;
+        const unsigned nt = ntiles / per_t;     // t-slices of tiles
+        const unsigned k = b / 8, x = b % 8;    // k-th workgroup of XCD x
+        const unsigned slab = 8 * (k / (nt * R)) + x;
+        const unsigned p = k % (nt * R);
+        const unsigned r = slab * R + p % R;    // tile within the t-slice
+        return r < per_t ? (p / R) * per_t + r : ntiles;
     }
     return b;
 }
-__host__ __device__ inline unsigned grid_for_tiles(unsigned ntiles, int mode, unsigned win) {
+__host__ __device__ inline unsigned grid_for_tiles(unsigned ntiles, int mode, unsigned win,
+                                                   unsigned per_t = 0u) {
     if (mode == 1) return 8 * ((ntiles + 7) / 8);
-    if (mode == 2) return ((ntiles + win - 1) / win) * win;
+    if (mode == 2) {
+        const unsigned w8 = win < 8u ? 8u : (win & ~7u);
+        return ((ntiles + w8 - 1) / w8) * w8;
+    }
+    if (mode == 3 && per_t > 0u) {
+        const unsigned R = win < 1u ? 1u : win;
+        const unsigned slabs = (per_t + R - 1) / R;
+        return 8 * ((slabs + 7) / 8) * (ntiles / per_t) * R;
+    }
     return ntiles;
 }
 
@@ -95,12 +135,23 @@ inline lattice make_lattice(const int dims[4]) {
         st *= (unsigned)dims[d];
     }
     g.V = st;
-    // measured best on MI355X at 32^4 and 48^4 (qcbf_autotune refines it)
-    g.map_mode = 2;
-    g.map_win = 1024;
     for (int d = 0; d < 4; d++) {
         g.lg_tile[d] = d == 0 ? -1 : 0;
         g.ntile[d] = 0;
+    }
+    // Slab sweep with 16 slabs per t-slice where the tiles allow it, else XCD
+    // chunks in windows of 1024 (measured on MI355X at 32^4, 48^4 and 64^4;
+    // qcbf_autotune refines it).
+    const unsigned pt = site_tiles_per_t(g);
+    if (pt && pt % 16 == 0) {
+        g.map_mode = 3;
+        g.map_win = pt / 16;
+    } else if (pt && pt % 8 == 0) {
+        g.map_mode = 3;
+        g.map_win = pt / 8;
+    } else {
+        g.map_mode = 2;
+        g.map_win = 1024;
     }
     return g;
 }
@@ -223,6 +274,19 @@ __device__ inline void store_link64(double2* __restrict__ p, unsigned V, int mu,
     double2* b = p + (size_t)mu * 9 * V + s;
 #pragma unroll
     for (int k = 0; k < 9; k++)
+        put2(b + (size_t)k * V, make_double2(m.e[k / 3][k % 3].r, m.e[k / 3][k % 3].i));
+}
+
+// The flow kernels store rows 0 and 1 only when every reader rebuilds the
+// third (QCB_F64_RECONSTRUCT): a third less write traffic per stage.  The
+// host completes the third row (launch_complete_rows) before the field
+// leaves the device.
+constexpr bool kF64FlowStoresTwoRows = QCB_F64_RECONSTRUCT != 0;
+__device__ inline void store_flow_link64(double2* __restrict__ p, unsigned V, int mu, unsigned s,
+                                         const mat3<double>& m) {
+    double2* b = p + (size_t)mu * 9 * V + s;
+#pragma unroll
+    for (int k = 0; k < (kF64FlowStoresTwoRows ? 6 : 9); k++)
         put2(b + (size_t)k * V, make_double2(m.e[k / 3][k % 3].r, m.e[k / 3][k % 3].i));
 }
 

@@ -237,7 +237,38 @@ __global__ __launch_bounds__(256) void k_cold_start(typename storage<P>::vec* __
     if (s >= V) return;
     mat3<typename storage<P>::real> m = zero3<typename storage<P>::real>();
     m.e[0][0].r = m.e[1][1].r = m.e[2][2].r = 1;
-    storage<P>::store(U, V, threadIdx.y, s, m);
+    if constexpr (P == prec::f64)
+        store_link64(U, V, threadIdx.y, s, m);
+    else
+        store_link32c(U, V, threadIdx.y, s, m);
+}
+
+// Third row of every FP64 link rebuilt from the first two, in place: makes a
+// flow output (two stored rows) a complete nine-entry field again.
+__global__ __launch_bounds__(256) void k_complete_rows64(double2* __restrict__ U, unsigned V) {
+    const unsigned s = blockIdx.x * kSitesPerBlock + threadIdx.x;
+    if (s >= V) return;
+    double2* b = U + (size_t)threadIdx.y * 9 * V + s;
+    cpx<double> m[2][3];
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const double2 v = b[(size_t)k * V];
+        m[k / 3][k % 3] = {v.x, v.y};
+    }
+    const cpx<double> r0 = conj(m[0][1] * m[1][2] - m[0][2] * m[1][1]);
+    const cpx<double> r1 = conj(m[0][2] * m[1][0] - m[0][0] * m[1][2]);
+    const cpx<double> r2 = conj(m[0][0] * m[1][1] - m[0][1] * m[1][0]);
+    b[(size_t)6 * V] = make_double2(r0.r, r0.i);
+    b[(size_t)7 * V] = make_double2(r1.r, r1.i);
+    b[(size_t)8 * V] = make_double2(r2.r, r2.i);
+}
+
+hipError_t launch_complete_rows(prec p, void* U, unsigned V, hipStream_t st) {
+    (void)hipGetLastError();  // drop a stale error of an earlier call
+    if (p != prec::f64 || !kF64FlowStoresTwoRows) return hipSuccess;
+    k_complete_rows64<<<(V + kSitesPerBlock - 1) / kSitesPerBlock, dim3(kSitesPerBlock, 4), 0, st>>>(
+        static_cast<double2*>(U), V);
+    return hipGetLastError();
 }
 
 hipError_t launch_hot_start(prec p, void* U, const lattice& g, uint64_t seed, hipStream_t st) {

@@ -25,7 +25,8 @@ __global__ __launch_bounds__(192) void k_measure(const typename storage<P>::vec*
     using FT = typename storage<P>::field;
     __shared__ double lds[3 * 3];
     const unsigned ntiles = measure_blocks(g);
-    const unsigned tile = tile_of_block(blockIdx.x, ntiles, g.map_mode, g.map_win);
+    const unsigned tile =
+        tile_of_block(blockIdx.x, ntiles, g.map_mode, g.map_win, site_tiles_per_t(g));
     if (tile >= ntiles) return;  // a whole padding workgroup: no barrier is skipped
     const unsigned s = tile * kSitesPerBlock + threadIdx.x;
     double v[3] = {0.0, 0.0, 0.0};
@@ -73,7 +74,7 @@ template <prec P>
 static hipError_t measure_launch(const void* U, const lattice& g, double* partials, double* out,
                                  bool clover, hipStream_t st) {
     const unsigned nblk = measure_blocks(g);
-    const unsigned grid = grid_for_tiles(nblk, g.map_mode, g.map_win);
+    const unsigned grid = grid_for_tiles(nblk, g.map_mode, g.map_win, site_tiles_per_t(g));
     const dim3 block(kSitesPerBlock, 3);
     const auto* u = static_cast<const typename storage<P>::vec*>(U);
     if (clover)

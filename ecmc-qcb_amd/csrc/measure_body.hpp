@@ -139,6 +139,25 @@ __device__ inline fclov<typename F::real> clover(const F& f, unsigned s, const n
     if constexpr (F::stream_rows) {
         C = clover_leaves_streamed<F, MU, NU>(f, s, xpmu, xpnu, xmmu, xmnu, xmmupnu, xmmumnu,
                                               xpmumnu, plaq);
+    } else if constexpr (QCB_SU3_PRODUCTS) {
+        // every partial product of a leaf is in SU(3): two rows + cross product
+        {
+            const mat3<T> b = mul_nd_su3(mul_su3(f.link(MU, s), f.link(NU, xpmu)), f.link(MU, xpnu));
+            C = mul_nd_su3(b, f.link(NU, s));
+            plaq += (double)(C.e[0][0].r + C.e[1][1].r + C.e[2][2].r);
+        }
+        {
+            const mat3<T> b =
+                mul_nd_su3(mul_nd_su3(f.link(NU, s), f.link(MU, xmmupnu)), f.link(NU, xmmu));
+            add_to(C, mul_su3(b, f.link(MU, xmmu)));
+        }
+        add_to(C, mul_dn_su3(mul_su3(f.link(NU, xmmumnu), f.link(MU, xmmu)),
+                             mul_su3(f.link(MU, xmmumnu), f.link(NU, xmnu))));
+        {
+            const mat3<T> b = mul_su3(mul_dn_su3(f.link(NU, xmnu), f.link(MU, xmnu)),
+                                      f.link(NU, xpmumnu));
+            add_to(C, mul_nd_su3(b, f.link(MU, s)));
+        }
     } else {
     {  // U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
         const mat3<T> a = mul(f.link(MU, s), f.link(NU, xpmu));
@@ -176,9 +195,14 @@ template <class F, int MU, int NU>
 __device__ inline void plaq_only(const F& f, unsigned s, const nbr& n, double& plaq) {
     using T = typename F::real;
     const unsigned xpmu = s + n.up[MU], xpnu = s + n.up[NU];
-    const mat3<T> a = mul(f.link(MU, s), f.link(NU, xpmu));
-    const mat3<T> b = mul_nd(a, f.link(MU, xpnu));
-    plaq += (double)re_tr_nd(b, f.link(NU, s));
+    if constexpr (QCB_SU3_PRODUCTS && !F::stream_rows) {
+        const mat3<T> b = mul_nd_su3(mul_su3(f.link(MU, s), f.link(NU, xpmu)), f.link(MU, xpnu));
+        plaq += (double)re_tr_nd(b, f.link(NU, s));
+    } else {
+        const mat3<T> a = mul(f.link(MU, s), f.link(NU, xpmu));
+        const mat3<T> b = mul_nd(a, f.link(MU, xpnu));
+        plaq += (double)re_tr_nd(b, f.link(NU, s));
+    }
 }
 
 // |F|_F^2 for anti-hermitian F

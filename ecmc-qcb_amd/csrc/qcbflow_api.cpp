@@ -35,6 +35,7 @@ struct qcbf_flow {
     hipStream_t stream = nullptr;
     void* U[2] = {nullptr, nullptr};  // ping-pong link fields
     int cur = 0;                      // U[cur] is the flowing field
+    bool rows_stale = false;          // U[cur] holds rows 0-1 only (a flow output)
     void* acc = nullptr;              // RK3 su(3) accumulator (8 reals / link)
     void* F[2] = {nullptr, nullptr};  // force_0, force_1 of the unfused API
     double* partials = nullptr;       // per-workgroup partial sums
@@ -224,6 +225,14 @@ static int ensure_stage(qcbf_flow* h) {
     return QCBF_OK;
 }
 
+// Complete the third rows of a flow output before the field is read whole.
+static int complete_rows(qcbf_flow* h) {
+    if (!h->rows_stale) return QCBF_OK;
+    HIPCHK(h, qcb::launch_complete_rows(h->prec, h->U[h->cur], h->g.V, h->stream));
+    h->rows_stale = false;
+    return QCBF_OK;
+}
+
 extern "C" int qcbf_set_links(qcbf_flow* h, const double* links) {
     CHECK_H(h);
     if (!links) return fail(h, QCBF_ERR_ARG, "null links");
@@ -237,6 +246,7 @@ extern "C" int qcbf_set_links(qcbf_flow* h, const double* links) {
                                            h->stream));
     }
     HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->rows_stale = false;
     return QCBF_OK;
 }
 
@@ -244,7 +254,7 @@ extern "C" int qcbf_get_links(qcbf_flow* h, double* links) {
     CHECK_H(h);
     if (!links) return fail(h, QCBF_ERR_ARG, "null links");
     int rc = ensure_stage(h);
-    if (rc) return rc;
+    if (rc || (rc = complete_rows(h))) return rc;
     double* st = static_cast<double*>(h->stage);
     for (unsigned s0 = 0; s0 < h->g.V; s0 += kChunkSites) {
         const unsigned n = std::min(kChunkSites, h->g.V - s0);
@@ -266,6 +276,7 @@ extern "C" int qcbf_copy_field(qcbf_flow* dst, qcbf_flow* src) {
     if (src == dst) return QCBF_OK;
     int rc = ensure_stage(dst);
     if (rc) return rc;
+    if ((rc = complete_rows(src))) return fail(dst, rc, "source: %s", src->err.c_str());
     // the source's queued work must be done before its field is read
     HIPCHK(dst, hipStreamSynchronize(src->stream));
     double* st = static_cast<double*>(dst->stage);
@@ -277,12 +288,15 @@ extern "C" int qcbf_copy_field(qcbf_flow* dst, qcbf_flow* src) {
                                              dst->stream));
     }
     HIPCHK(dst, hipStreamSynchronize(dst->stream));
+    dst->rows_stale = false;
     return QCBF_OK;
 }
 
 // The ensemble ABI copies from a single-lattice handle through this view.
 int qcb::field_view(qcbf_flow* h, int* prec, const void** field, hipStream_t* stream) {
     if (!h) return QCBF_ERR_ARG;
+    const int rc = complete_rows(h);  // the field is read whole
+    if (rc) return rc;
     *prec = (int)h->prec;
     *field = h->U[h->cur];
     *stream = h->stream;
@@ -292,6 +306,7 @@ int qcb::field_view(qcbf_flow* h, int* prec, const void** field, hipStream_t* st
 extern "C" int qcbf_hot_start(qcbf_flow* h, uint64_t seed) {
     CHECK_H(h);
     HIPCHK(h, qcb::launch_hot_start(h->prec, h->U[h->cur], h->g, seed, h->stream));
+    h->rows_stale = false;
     return QCBF_OK;
 }
 
@@ -308,6 +323,7 @@ extern "C" int qcbf_heatbath(qcbf_flow* h, double beta, int n_sweeps, uint64_t s
     HIPCHK(h, qcb::launch_heatbath_f64(static_cast<double2*>(h->U[h->cur]), h->g, beta, n_sweeps,
                                        seed, h->hb_sweeps, h->dwarn, h->stream));
     h->hb_sweeps += (unsigned)n_sweeps;
+    if (n_sweeps > 0) h->rows_stale = false;  // every link rewritten whole
     int stuck = 0;
     HIPCHK(h, hipMemcpyAsync(&stuck, h->dwarn, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipMemsetAsync(h->dwarn, 0, sizeof(int), h->stream));
@@ -319,6 +335,7 @@ extern "C" int qcbf_heatbath(qcbf_flow* h, double beta, int n_sweeps, uint64_t s
 extern "C" int qcbf_cold_start(qcbf_flow* h) {
     CHECK_H(h);
     HIPCHK(h, qcb::launch_cold_start(h->prec, h->U[h->cur], h->g, h->stream));
+    h->rows_stale = false;
     return QCBF_OK;
 }
 
@@ -398,6 +415,7 @@ extern "C" int qcbf_load_ildg(qcbf_flow* h, const char* path) {
         k ^= 1;
     }
     HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->rows_stale = false;
     return QCBF_OK;
 }
 
@@ -405,7 +423,7 @@ extern "C" int qcbf_save_ildg(qcbf_flow* h, const char* path) {
     CHECK_H(h);
     if (!path) return fail(h, QCBF_ERR_ARG, "null path");
     int rc = ensure_stage(h);
-    if (rc) return rc;
+    if (rc || (rc = complete_rows(h))) return rc;
     FILE* fp = fopen(path, "wb");
     if (!fp) return fail(h, QCBF_ERR_IO, "cannot open '%s' for writing", path);
     file_closer fc{fp};
@@ -447,6 +465,7 @@ static int rk3_once(qcbf_flow* h, double eps) {
         if (rc) return rc;
         h->cur ^= 1;
     }
+    h->rows_stale = h->prec == qcb::prec::f64 && qcb::kF64FlowStoresTwoRows;
     return QCBF_OK;
 }
 
@@ -479,6 +498,7 @@ static int update(qcbf_flow* h, int which, double eps) {
         return fail(h, QCBF_ERR_STATE, "compute_w%d needs the forces it combines", which);
     HIPCHK(h, qcb::launch_update(h->prec, which, h->U[h->cur], h->F[0], h->F[1], h->g, eps,
                                  h->dwarn, h->stream));
+    h->rows_stale = h->prec == qcb::prec::f64 && qcb::kF64FlowStoresTwoRows;
     return QCBF_OK;
 }
 extern "C" int qcbf_compute_w1(qcbf_flow* h, double eps) {
@@ -665,12 +685,12 @@ extern "C" int qcbf_tune(qcbf_flow* h, int key, long value) {
     CHECK_H(h);
     switch (key) {
         case QCBF_TUNE_MAP_MODE:
-            if (value < 0 || value > 2) return fail(h, QCBF_ERR_ARG, "map mode %ld not in 0..2", value);
+            if (value < 0 || value > 3) return fail(h, QCBF_ERR_ARG, "map mode %ld not in 0..3", value);
             h->g.map_mode = (int)value;
             return QCBF_OK;
         case QCBF_TUNE_MAP_WIN:
-            if (value < 8 || value % 8 != 0 || value > (1L << 24))
-                return fail(h, QCBF_ERR_ARG, "map window %ld not a multiple of 8 in [8, 2^24]", value);
+            if (value < 1 || value > (1L << 24))
+                return fail(h, QCBF_ERR_ARG, "map window %ld not in [1, 2^24]", value);
             h->g.map_win = (unsigned)value;
             return QCBF_OK;
         case QCBF_TUNE_TILE_SHAPE: {
@@ -799,7 +819,7 @@ extern "C" int qcbf_autotune(qcbf_flow* h, double eps, int trial_steps, long* mo
         return fail(h, QCBF_ERR_OOM, "autotune save buffer: %s", hipGetErrorString(e));
     }
     const int cur0 = h->cur;
-    const bool timing0 = h->timing;
+    const bool timing0 = h->timing, stale0 = h->rows_stale;
     h->timing = false;
     struct cand {
         int mode;
@@ -810,12 +830,16 @@ extern "C" int qcbf_autotune(qcbf_flow* h, double eps, int trial_steps, long* mo
     const std::pair<int, unsigned> maps[] = {{0, 64}, {1, 64}, {2, 256}, {2, 1024}, {2, 4096}};
     const int shapes[][4] = {{-1, 0, 0, 0}, {2, 2, 1, 1}, {3, 2, 1, 0},
                              {3, 3, 0, 0},  {4, 2, 0, 0}, {2, 2, 2, 0}};
-    for (const auto& m : maps)
-        for (const auto& sh : shapes) {
-            qcb::lattice probe = h->g;
-            if (sh[0] >= 0 && !qcb::set_tile_shape(probe, sh)) continue;
-            cands.push_back({m.first, m.second, {sh[0], sh[1], sh[2], sh[3]}});
-        }
+    for (const auto& sh : shapes) {
+        qcb::lattice probe = h->g;
+        if (sh[0] >= 0 && !qcb::set_tile_shape(probe, sh)) continue;
+        if (sh[0] < 0) qcb::set_tile_shape(probe, nullptr);
+        for (const auto& m : maps) cands.push_back({m.first, m.second, {sh[0], sh[1], sh[2], sh[3]}});
+        // slab sweeps: 8, 16 or 32 slabs per t-slice (one, two or four per XCD)
+        const unsigned pt = qcb::stage_tiles_per_t(probe);
+        for (unsigned ns : {8u, 16u, 32u})
+            if (pt && pt % ns == 0) cands.push_back({3, pt / ns, {sh[0], sh[1], sh[2], sh[3]}});
+    }
     const qcb::lattice g0 = h->g;
     cand best = {g0.map_mode, g0.map_win, {g0.lg_tile[0], g0.lg_tile[1], g0.lg_tile[2], g0.lg_tile[3]}};
     float best_ms = 1e30f;
@@ -855,6 +879,7 @@ extern "C" int qcbf_autotune(qcbf_flow* h, double eps, int trial_steps, long* mo
     qcb::set_tile_shape(h->g, best.lg[0] < 0 ? nullptr : best.lg);
     h->cur = cur0;
     h->timing = timing0;
+    h->rows_stale = stale0;
     if ((e = hipMemcpyAsync(h->U[cur0], saveU, ub, hipMemcpyDeviceToDevice, h->stream)) != hipSuccess ||
         (e = hipMemcpyAsync(h->acc, saveA, ab, hipMemcpyDeviceToDevice, h->stream)) != hipSuccess ||
         (e = hipStreamSynchronize(h->stream)) != hipSuccess)

@@ -34,6 +34,7 @@ struct qcbf_batch {
     hipStream_t stream = nullptr;
     char* U[2] = {nullptr, nullptr};
     int cur = 0;
+    std::vector<char> rows_stale;  // per configuration: field holds rows 0-1 only
     char* acc = nullptr;
     cfg_desc* d_cfgs = nullptr;
     unsigned* d_tile_cfg = nullptr;
@@ -138,6 +139,7 @@ extern "C" int qcbf_batch_create(int n, const int* dims, int precision, qcbf_bat
     }
     qcbf_batch* b = new qcbf_batch();
     b->n = n;
+    b->rows_stale.assign(n, 0);
     b->precision = precision;
     b->prec = precision == QCBF_FP64 ? qcb::prec::f64 : qcb::prec::f32m;
     const size_t lw_per_site = precision == QCBF_FP64 ? 36 : 12;  // link-field words per site
@@ -272,7 +274,14 @@ extern "C" int qcbf_batch_set_links(qcbf_batch* b, int c, const double* links) {
         BCHK(b, qcb::launch_aos_to_field(b->prec, b->stage, 0, field_of(b, c), g, s0, n, b->stream));
     }
     BCHK(b, hipStreamSynchronize(b->stream));
+    b->rows_stale[c] = 0;
     return QCBF_OK;
+}
+
+// Flow steps store FP64 links as two rows; mark every configuration.
+static void mark_flowed(qcbf_batch* b) {
+    const bool two = b->prec == qcb::prec::f64 && qcb::kF64FlowStoresTwoRows;
+    for (auto& f : b->rows_stale) f = two ? 1 : 0;
 }
 
 extern "C" int qcbf_batch_get_links(qcbf_batch* b, int c, double* links) {
@@ -282,6 +291,10 @@ extern "C" int qcbf_batch_get_links(qcbf_batch* b, int c, double* links) {
     if (!links) return bfail(b, QCBF_ERR_ARG, "null links");
     if ((rc = ensure_stage(b))) return rc;
     const qcb::lattice& g = b->cfgs[c].g;
+    if (b->rows_stale[c]) {
+        BCHK(b, qcb::launch_complete_rows(b->prec, field_of(b, c), g.V, b->stream));
+        b->rows_stale[c] = 0;
+    }
     double* st = static_cast<double*>(b->stage);
     for (unsigned s0 = 0; s0 < g.V; s0 += kChunkSites) {
         const unsigned n = std::min(kChunkSites, g.V - s0);
@@ -298,6 +311,7 @@ extern "C" int qcbf_batch_hot_start(qcbf_batch* b, int c, uint64_t seed) {
     int rc = check_cfg(b, c);
     if (rc) return rc;
     BCHK(b, qcb::launch_hot_start(b->prec, field_of(b, c), b->cfgs[c].g, seed, b->stream));
+    b->rows_stale[c] = 0;
     return QCBF_OK;
 }
 
@@ -324,6 +338,7 @@ extern "C" int qcbf_batch_copy_from(qcbf_batch* b, int c, qcbf_flow* src) {
         BCHK(b, qcb::launch_aos_to_field(b->prec, st, 0, field_of(b, c), g, s0, n, b->stream));
     }
     BCHK(b, hipStreamSynchronize(b->stream));
+    b->rows_stale[c] = 0;
     return QCBF_OK;
 }
 
@@ -349,6 +364,7 @@ static int enqueue_measure(qcbf_batch* b, double* out, unsigned* slot, bool bump
 extern "C" int qcbf_batch_rk3_step(qcbf_batch* b, double eps, int n_steps) {
     CHECK_B(b);
     if (n_steps < 0) return bfail(b, QCBF_ERR_ARG, "n_steps < 0");
+    if (n_steps > 0) mark_flowed(b);
     for (int k = 0; k < n_steps; k++) {
         int rc = enqueue_step(b, eps, b->cur);
         if (rc) return rc;
@@ -414,6 +430,7 @@ extern "C" int qcbf_batch_flow_measure(qcbf_batch* b, double eps, int n_steps_gf
         b->dres_cap = need;
         drop_graphs(b);  // graphs hold the old results pointer
     }
+    if (n_steps_gf > 1 && n_rk_steps > 0) mark_flowed(b);
     if (b->timing) BCHK(b, hipEventRecord(b->ev[0], b->stream));
     BCHK(b, qcb::launch_set_counter(b->d_slot, 0, b->stream));
     std::vector<double> tvals(n_steps_gf);
@@ -563,6 +580,7 @@ extern "C" int qcbf_batch_autotune(qcbf_batch* b, double eps, int trial_steps, l
         return bfail(b, QCBF_ERR_OOM, "autotune save buffer: %s", hipGetErrorString(e));
     }
     const int cur0 = b->cur;
+    const std::vector<char> stale0 = b->rows_stale;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = QCBF_OK;
     struct cand {
@@ -630,6 +648,7 @@ extern "C" int qcbf_batch_autotune(qcbf_batch* b, double eps, int trial_steps, l
     if ((e = apply_shape(best.shape)) != hipSuccess && rc == QCBF_OK)
         rc = bfail(b, QCBF_ERR_HIP, "autotune shape: %s", hipGetErrorString(e));
     b->cur = cur0;
+    b->rows_stale = stale0;
     if ((e = hipMemcpyAsync(b->U[cur0], saveU, ub, hipMemcpyDeviceToDevice, b->stream)) != hipSuccess ||
         (e = hipMemcpyAsync(b->acc, saveA, ab, hipMemcpyDeviceToDevice, b->stream)) != hipSuccess ||
         (e = hipStreamSynchronize(b->stream)) != hipSuccess)

@@ -172,6 +172,76 @@ __host__ __device__ inline void acc_mul_dn(mat3<T>& R, const mat3<T>& A, const m
             R.e[i][j] = s;
         }
 }
+// ---- products of two SU(3) matrices --------------------------------------------
+// The product of two SU(3) matrices is in SU(3), so its third row is
+// conj(row0 x row1) (U^dag = adj U when det U = 1).  The *_su3 products form
+// rows 0 and 1 (18 complex multiply-adds) and the cross product (6 complex
+// products) instead of the 27 multiply-adds of the third row, and they read
+// only rows 0-1 of a left operand that enters by rows, which lets the caller
+// skip rebuilding that operand's third row.  Equal to the full product up to
+// rounding of the order of the operands' distance from SU(3).
+template <typename T>
+__host__ __device__ inline void complete_su3(mat3<T>& m) {
+    m.e[2][0] = conj(m.e[0][1] * m.e[1][2] - m.e[0][2] * m.e[1][1]);
+    m.e[2][1] = conj(m.e[0][2] * m.e[1][0] - m.e[0][0] * m.e[1][2]);
+    m.e[2][2] = conj(m.e[0][0] * m.e[1][1] - m.e[0][1] * m.e[1][0]);
+}
+// A B, A and B in SU(3) (reads rows 0-1 of A)
+template <typename T>
+__host__ __device__ inline mat3<T> mul_su3(const mat3<T>& A, const mat3<T>& B) {
+    mat3<T> R;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            cpx<T> s = A.e[i][0] * B.e[0][j];
+            s = madd(s, A.e[i][1], B.e[1][j]);
+            s = madd(s, A.e[i][2], B.e[2][j]);
+            R.e[i][j] = s;
+        }
+    complete_su3(R);
+    return R;
+}
+// A B^dag, A and B in SU(3) (reads rows 0-1 of A)
+template <typename T>
+__host__ __device__ inline mat3<T> mul_nd_su3(const mat3<T>& A, const mat3<T>& B) {
+    mat3<T> R;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            cpx<T> s = A.e[i][0] * conj(B.e[j][0]);
+            s = madd_c(s, A.e[i][1], B.e[j][1]);
+            s = madd_c(s, A.e[i][2], B.e[j][2]);
+            R.e[i][j] = s;
+        }
+    complete_su3(R);
+    return R;
+}
+// A^dag B, A and B in SU(3)
+template <typename T>
+__host__ __device__ inline mat3<T> mul_dn_su3(const mat3<T>& A, const mat3<T>& B) {
+    mat3<T> R;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            cpx<T> s = conj(A.e[0][i]) * B.e[0][j];
+            s = madd_cl(s, A.e[1][i], B.e[1][j]);
+            s = madd_cl(s, A.e[2][i], B.e[2][j]);
+            R.e[i][j] = s;
+        }
+    complete_su3(R);
+    return R;
+}
+template <typename T>
+__host__ __device__ inline void add_to(mat3<T>& R, const mat3<T>& A) {
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) R.e[i][j] = R.e[i][j] + A.e[i][j];
+}
+
 // Re tr(A B^dag) = sum_ij Re(A_ij conj(B_ij))
 template <typename T>
 __host__ __device__ inline T re_tr_nd(const mat3<T>& A, const mat3<T>& B) {

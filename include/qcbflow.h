@@ -175,8 +175,11 @@ int qcbf_su3_exp(const double* Z, int n, double coeff, double* out, int* flagged
 enum qcbf_tune_key {
     QCBF_TUNE_MAP_MODE = 0, /* workgroup -> site-tile order of the flow kernels:
                                0 linear, 1 one contiguous chunk per XCD,
-                               2 XCD chunks inside windows of MAP_WIN workgroups */
-    QCBF_TUNE_MAP_WIN = 1,  /* window for mode 2: a multiple of 8, >= 8 */
+                               2 XCD chunks inside windows of MAP_WIN workgroups,
+                               3 slab sweep: each XCD walks slabs of MAP_WIN tiles
+                                 of a t-slice through t (single lattices only) */
+    QCBF_TUNE_MAP_WIN = 1,  /* mode 2: window (rounded down to a multiple of 8, at
+                               least 8); mode 3: slab width in tiles; >= 1 */
     QCBF_TUNE_BLOCKS_PER_CU = 2, /* ensembles only: persistent-grid workgroups per CU */
     QCBF_TUNE_TILE_SHAPE = 3     /* sites of one flow workgroup: -1 = 64 consecutive
                                     sites, else a 4-D box of extents 2^a x 2^b x 2^c x 2^d

@@ -111,7 +111,8 @@ def test_single_lattice_autotune_and_tile_maps_agree():
         shapes = [-1, qcbflow.tile_code(4, 4, 2, 2), qcbflow.tile_code(8, 4, 2, 1),
                   qcbflow.tile_code(8, 8, 1, 1), qcbflow.tile_code(2, 2, 4, 4)]
         for mode, win, shape in [(0, 64, -1), (1, 64, shapes[1]), (2, 64, shapes[2]),
-                                 (2, 1024, shapes[3]), (0, 64, shapes[4])]:
+                                 (2, 1024, shapes[3]), (0, 64, shapes[4]), (3, 2, -1),
+                                 (3, 1, shapes[1]), (3, 3, shapes[2]), (2, 20, -1)]:
             gf.tune(qcbflow.TUNE_MAP_MODE, mode)
             gf.tune(qcbflow.TUNE_MAP_WIN, win)
             gf.tune(qcbflow.TUNE_TILE_SHAPE, shape)

@@ -214,3 +214,39 @@ def test_large_lattice_properties_32x32x32x32():
         gf.copy(fields.gauge_apply(g, L, G))
         p3, e3, q3 = gf.measure()
         assert abs(p3 - p0) < 1e-13 and abs(e3 - e0) < 1e-12 * e0 and abs(q3 - q0) < 1e-10
+
+
+def _su3_defects(f):
+    """max |U U^dag - 1| and max |det U - 1| over all links of a host field."""
+    U = fields.to_cplx(f).reshape(-1, 3, 3)
+    uu = np.einsum("nij,nkj->nik", U, U.conj()) - np.eye(3)
+    return np.abs(uu).max(), np.abs(np.linalg.det(U) - 1).max()
+
+
+def test_flow_output_leaves_device_whole():
+    """The FP64 flow stores two rows per link; every path that hands the field
+    out (get_links, ILDG save, device copies, ensembles) rebuilds the third,
+    so what leaves the device is SU(3) to rounding and agrees with the oracle,
+    and a field that was never flowed leaves bit-for-bit as it came in."""
+    L = (6, 4, 4, 8)
+    f = thermal(L, 77, 2)
+    ref = f.copy()
+    oracle.rk3(ref, L, 0.02, 3)
+    with qcbflow.GradientFlow(L) as gf, qcbflow.GradientFlow(L) as other:
+        gf.copy(f)
+        assert np.array_equal(gf.field(), f)
+        gf.rk3_step(3)
+        out = gf.field()
+        du, dd = _su3_defects(out)
+        assert du < 1e-14 and dd < 1e-14
+        assert np.abs(out - ref).max() < 1e-13
+        gf.rk3_step(2)
+        other.copy_from(gf)  # source completed on the device before the copy
+        g = gf.field()
+        assert np.array_equal(other.field(), g)
+        assert max(_su3_defects(g)) < 1e-14
+    with qcbflow.Ensemble([L, (4, 4, 4, 4)]) as ens:
+        ens.copy(0, f)
+        ens.rk3_step(3)
+        e0 = ens.field(0)
+        assert np.array_equal(e0, out)
