@@ -429,6 +429,19 @@ def main():
     gf.synchronize()
     flow_only = V * K / (time.perf_counter() - t2)
 
+    # the same series when the field comes from and goes back to host memory
+    gf.synchronize()
+    t3 = time.perf_counter()
+    host_links = src.field()
+    t_down = time.perf_counter() - t3
+    t3 = time.perf_counter()
+    gf.copy(host_links)
+    gf.synchronize()
+    t_up = time.perf_counter() - t3
+    pcie_value = V * K / (secs + t_up + t_down)
+    log(f"host transfers of the field: up {t_up * 1e3:.1f} ms, down {t_down * 1e3:.1f} ms -> "
+        f"{pcie_value:.4e} site-steps/s including them")
+
     copy_ms = None
     if args.calibrate_copy_mib > 0:
         copy_ms = gf.copy_bench(args.calibrate_copy_mib << 20, 10)
@@ -463,7 +476,7 @@ def main():
 
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
-        links = src.field()
+        links = host_links
         threads = min(args.cpu_threads, os.cpu_count() or 1)
         log(f"cpu baseline: {args.cpu_steps} steps on {threads} threads ...")
         cpu = cpu_baseline(links, L, args.eps, args.cpu_steps, threads)
@@ -513,6 +526,8 @@ def main():
         },
         "measure_avg_ms": meas_avg_ms,
         "flow_only_value": flow_only,
+        "pcie_inclusive_value": pcie_value,
+        "host_transfer_ms": {"up": t_up * 1e3, "down": t_down * 1e3},
         "fp32_mixed": mixed,
         "cpu_baseline": cpu,
     }

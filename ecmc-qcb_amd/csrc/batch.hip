@@ -135,9 +135,8 @@ __global__ __launch_bounds__(256) void k_batch_reduce(const double* __restrict__
     __shared__ double lds[4 * 3];
     const cfg_desc& c = cfgs[blockIdx.x];
     double v[3] = {0.0, 0.0, 0.0}, comp[3] = {0.0, 0.0, 0.0};
-    for (unsigned t = threadIdx.x; t < c.ntiles; t += blockDim.x)
-#pragma unroll
-        for (int k = 0; k < 3; k++) neumaier_add(v[k], comp[k], partials[3 * (c.tile0 + t) + k]);
+    strided_compensated_sum<3>(partials + 3 * (size_t)c.tile0, c.ntiles, threadIdx.x, blockDim.x, v,
+                               comp);
 #pragma unroll
     for (int k = 0; k < 3; k++) v[k] += comp[k];
     block_sum<3>(v, lds, 4);

@@ -56,10 +56,7 @@ __global__ __launch_bounds__(256) void k_reduce_measure(const double* __restrict
                                                         double* __restrict__ out) {
     __shared__ double lds[4 * 3];
     double v[3] = {0.0, 0.0, 0.0}, c[3] = {0.0, 0.0, 0.0};
-    for (unsigned b = threadIdx.x; b < nblk; b += blockDim.x) {
-#pragma unroll
-        for (int k = 0; k < 3; k++) neumaier_add(v[k], c[k], partials[3 * b + k]);
-    }
+    strided_compensated_sum<3>(partials, nblk, threadIdx.x, blockDim.x, v, c);
 #pragma unroll
     for (int k = 0; k < 3; k++) v[k] += c[k];
     block_sum<3>(v, lds, 4);

@@ -235,10 +235,4 @@ __device__ inline void pair_body(const F& f, unsigned s, const nbr& n, double& v
     }
 }
 
-__device__ inline void neumaier_add(double& sum, double& comp, double x) {
-    const double t = sum + x;
-    comp += (fabs(sum) >= fabs(x)) ? (sum - t) + x : (x - t) + sum;
-    sum = t;
-}
-
 }  // namespace qcb

@@ -45,4 +45,37 @@ __device__ inline void block_sum(double (&v)[NV], double* lds, int nwaves) {
     }
 }
 
+// Neumaier's compensated summation step (Z. Angew. Math. Mech. 54 (1974) 39).
+__device__ inline void neumaier_add(double& sum, double& comp, double x) {
+    const double t = sum + x;
+    comp += (fabs(sum) >= fabs(x)) ? (sum - t) + x : (x - t) + sum;
+    sum = t;
+}
+
+// Compensated sums over i = t, t + stride, t + 2 stride, ... < n of the NV
+// values p[NV*i + k], in that order.  The loads of 8 consecutive terms are
+// issued before their additions, so the loop waits on memory once per 8
+// terms instead of once per term; the order of the additions is unchanged.
+template <int NV>
+__device__ inline void strided_compensated_sum(const double* __restrict__ p, unsigned n,
+                                               unsigned t, unsigned stride, double (&v)[NV],
+                                               double (&c)[NV]) {
+    constexpr unsigned kB = 8;
+    unsigned i = t;
+    for (; i + (kB - 1) * stride < n; i += kB * stride) {
+        double x[kB][NV];
+#pragma unroll
+        for (unsigned j = 0; j < kB; j++)
+#pragma unroll
+            for (int k = 0; k < NV; k++) x[j][k] = p[NV * (i + j * stride) + k];
+#pragma unroll
+        for (unsigned j = 0; j < kB; j++)
+#pragma unroll
+            for (int k = 0; k < NV; k++) neumaier_add(v[k], c[k], x[j][k]);
+    }
+    for (; i < n; i += stride)
+#pragma unroll
+        for (int k = 0; k < NV; k++) neumaier_add(v[k], c[k], p[NV * i + k]);
+}
+
 }  // namespace qcb
