@@ -48,6 +48,10 @@ MEASURE_FLOPS_PER_SITE = 15150.0
 #   third-row reconstruction.
 MIXED_BYTES_PER_SITE_STEP = 1664.0
 MIXED_FLOPS_PER_SITE_STEP = 40956.0
+# per stage (1, 2, 3): links read + written, accumulator written / read+written / read
+STAGE_BYTES = {"fp64": (1408.0, 1664.0, 1408.0), "fp32_mixed": (512.0, 640.0, 512.0)}
+# one read of the links by the measurement
+MEASURE_BYTES = {"fp64": 576.0, "fp32_mixed": 192.0}
 
 
 def log(msg):
@@ -155,11 +159,26 @@ def compare(t64, p64, t32, p32, t_gate=2.0):
     }
 
 
-def stage_roofline(ms, launches, V, bytes_per_site_step):
-    stage_n = launches[0] + launches[1] + launches[2]
-    avg = (ms[0] + ms[1] + ms[2]) / max(1, stage_n)
-    gbs = V * bytes_per_site_step / 3.0 / (avg * 1e-3) / 1e9
-    return avg, gbs
+def stage_roofline(ms, launches, V, tag):
+    """Average duration (ms) and algorithmic GB/s of the plain stage kernels
+    (timing classes 0-2; with the fused measurement stage 1 runs as class 4),
+    bytes weighted by stage."""
+    sb = STAGE_BYTES[tag]
+    n = launches[0] + launches[1] + launches[2]
+    t = ms[0] + ms[1] + ms[2]
+    byt = V * sum(sb[k] * launches[k] for k in range(3))
+    avg = t / max(1, n)
+    return avg, (byt / (t * 1e-3) / 1e9 if t > 0 else 0.0), byt / max(1, n)
+
+
+def fused_stats(ms, launches, V, tag):
+    """The fused stage 1 + measurement kernel (timing class 4), or None."""
+    if launches[4] == 0:
+        return None
+    avg = ms[4] / launches[4]
+    byt = V * (STAGE_BYTES[tag][0] + MEASURE_BYTES[tag])
+    return {"avg_ms": avg, "launches": launches[4], "bytes_per_launch": byt,
+            "achieved_gbs": byt / (avg * 1e-3) / 1e9}
 
 
 def setup_shape(qcbflow, h, args):
@@ -171,8 +190,10 @@ def setup_shape(qcbflow, h, args):
             h.tune(key, val)
     if not args.no_autotune:
         h.autotune(trial_steps=2)
+    if args.no_fuse:
+        h.tune(qcbflow.TUNE_FUSED_MEASURE, 0)
     return [h.get_tune(k) for k in (qcbflow.TUNE_MAP_MODE, qcbflow.TUNE_MAP_WIN,
-                                    qcbflow.TUNE_TILE_SHAPE)]
+                                    qcbflow.TUNE_TILE_SHAPE, qcbflow.TUNE_FUSED_MEASURE)]
 
 
 def lattice_of(args, default):
@@ -204,7 +225,7 @@ def report_single(qcbflow, args, name, L, beta, seed, sweeps, K, every):
         h.copy_from(src)
         shape = setup_shape(qcbflow, h, args)
         secs, tqe, plaq, ms, n = timed_flow(h, src, K, args.warmup, every)
-        avg, gbs = stage_roofline(ms, n, V, bps)
+        avg, gbs, _ = stage_roofline(ms, n, V, tag)
         legs[tag] = {
             "value": V * K / secs,
             "ms_per_rk3_step": secs / K * 1e3,
@@ -213,6 +234,7 @@ def report_single(qcbflow, args, name, L, beta, seed, sweeps, K, every):
             "hbm_frac": gbs / HBM_PEAK_GBS,
             "bytes_per_site_step": bps,
             "measure_avg_ms": ms[3] / max(1, n[3]),
+            "stage1_measure": fused_stats(ms, n, V, tag),
             "launch_shape": shape,
             "exp_warnings": h.exp_warnings(),
         }
@@ -305,6 +327,8 @@ def report_ensemble(qcbflow, args):
     for p, tag in ((qcbflow.FP64, "fp64"), (qcbflow.FP32_MIXED, "fp32_mixed")):
         e = ens[p]
         e.use_graph(True)
+        if args.no_fuse:
+            e.tune(qcbflow.TUNE_FUSED_MEASURE, 0)
         m, w, b = (None, None, None)
         if not args.no_autotune:
             m, w, b = e.autotune(trial_steps=2)
@@ -364,6 +388,8 @@ def main():
     ap.add_argument("--tile-shape", type=int, default=None, help="qcbf_tune tile-shape code")
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip qcbf_autotune before the warmup")
+    ap.add_argument("--no-fuse", action="store_true",
+                    help="measure with a separate kernel instead of inside the next RK3 stage 1")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run the plain copy kernel on this many MiB (traffic calibration)")
     args = ap.parse_args()
@@ -415,9 +441,8 @@ def main():
     log(f"timed: {K} steps in {secs:.4f} s -> {value:.4e} site-steps/s; "
         f"E(t={tqe[-1, 0]:.2f}) = {tqe[-1, 2]:.6g}, Q = {tqe[-1, 1]:.6g}")
 
-    avg_stage_ms, achieved = stage_roofline(ms, launches, V, FLOW_BYTES_PER_SITE_STEP)
-    bytes_per_launch = V * FLOW_BYTES_PER_SITE_STEP / 3.0
-    per_stage = {f"stage{k + 1}_avg_ms": ms[k] / max(1, launches[k]) for k in range(3)}
+    avg_stage_ms, achieved, bytes_per_launch = stage_roofline(ms, launches, V, "fp64")
+    per_stage = {f"stage{k + 1}_avg_ms": ms[k] / launches[k] for k in range(3) if launches[k]}
     meas_avg_ms = ms[3] / max(1, launches[3])
     flops_rate = V * FLOW_FLOPS_PER_SITE_STEP / (3 * avg_stage_ms * 1e-3) / 1e12
     traffic = traffic_from_profiles(L)
@@ -456,13 +481,14 @@ def main():
         gm.copy_from(src)
         shape32 = setup_shape(qcbflow, gm, args)
         msecs, mtqe, mplaq, mms, mn = timed_flow(gm, src, K, args.warmup)
-        mstage, mach = stage_roofline(mms, mn, V, MIXED_BYTES_PER_SITE_STEP)
+        mstage, mach, _ = stage_roofline(mms, mn, V, "fp32_mixed")
         mixed = {
             "value": V * K / msecs,
             "vs_fp64": (V * K / msecs) / value,
             "ms_per_step": msecs / K * 1e3,
             "stage_avg_ms": mstage,
             "measure_avg_ms": mms[3] / max(1, mn[3]),
+            "stage1_measure": fused_stats(mms, mn, V, "fp32_mixed"),
             "hbm_achieved_gbs": mach,
             "hbm_frac": mach / HBM_PEAK_GBS,
             "bytes_per_site_step": MIXED_BYTES_PER_SITE_STEP,
@@ -525,6 +551,7 @@ def main():
             **per_stage,
         },
         "measure_avg_ms": meas_avg_ms,
+        "stage1_measure": fused_stats(ms, launches, V, "fp64"),
         "flow_only_value": flow_only,
         "pcie_inclusive_value": pcie_value,
         "host_transfer_ms": {"up": t_up * 1e3, "down": t_down * 1e3},

@@ -89,42 +89,105 @@ hipError_t launch_batch_stage(prec p, int stage, const void* Uin, void* Uout, vo
 }
 
 // ---- measurement ---------------------------------------------------------------
+// Same per-site code and workgroup reduction as k_measure_clover (measure.hip):
+// wavefront mu forms its clover leaves, LDS joins the two halves of each plane.
 template <prec P>
-__global__ __launch_bounds__(192) void k_batch_measure(const typename storage<P>::vec* __restrict__ U,
+__global__ __launch_bounds__(256) void k_batch_measure(const typename storage<P>::vec* __restrict__ U,
                                                        const cfg_desc* __restrict__ cfgs,
                                                        const unsigned* __restrict__ tile_cfg,
                                                        unsigned ntiles, int map_mode,
                                                        unsigned map_win,
                                                        double* __restrict__ partials) {
     using FT = typename storage<P>::field;
-    __shared__ double lds[3 * 3];
+    __shared__ typename FT::real lds[kCloverSlots * 9 * 64];
+    __shared__ double red[4 * 3];
     const unsigned nslots = grid_for_tiles(ntiles, map_mode, map_win);
-    for (unsigned v0 = 0; v0 < nslots; v0 += gridDim.x) {
-        // every workgroup runs the same number of iterations (block_sum syncs)
-        const unsigned v = v0 + blockIdx.x;
-        const unsigned tile = v < nslots ? tile_of_block(v, ntiles, map_mode, map_win) : ntiles;
+    const unsigned lane = threadIdx.x;
+    for (unsigned v0 = blockIdx.x; v0 < nslots; v0 += gridDim.x) {  // uniform per workgroup
+        const unsigned tile = tile_of_block(v0, ntiles, map_mode, map_win);
+        if (tile >= ntiles) continue;
+        const cfg_desc& c = cfgs[tile_cfg[tile]];
+        const unsigned s = (tile - c.tile0) * kSitesPerBlock + lane;
+        const bool valid = s < c.g.V;
+        const FT f = storage<P>::view(U + c.u_off, c.g.V);
+        const nbr n = neighbours(c.g, valid ? s : 0u);
         double r[3] = {0.0, 0.0, 0.0};
-        if (tile < ntiles) {
-            const cfg_desc& c = cfgs[tile_cfg[tile]];
-            const unsigned s = (tile - c.tile0) * kSitesPerBlock + threadIdx.x;
-            if (s < c.g.V) {
-                const nbr n = neighbours(c.g, s);
-                const FT f = storage<P>::view(U + c.u_off, c.g.V);
-                switch (threadIdx.y) {
-                    case 0: pair_body<FT, 0, 1, 2, 3, +1, true>(f, s, n, r[0], r[1], r[2]); break;
-                    case 1: pair_body<FT, 0, 2, 1, 3, -1, true>(f, s, n, r[0], r[1], r[2]); break;
-                    default: pair_body<FT, 0, 3, 1, 2, +1, true>(f, s, n, r[0], r[1], r[2]); break;
-                }
+        if (valid) {
+            switch (threadIdx.y) {
+                case 0: r[0] = clover_wave<FT, 0>(f, s, n, lds, lane); break;
+                case 1: r[0] = clover_wave<FT, 1>(f, s, n, lds, lane); break;
+                case 2: r[0] = clover_wave<FT, 2>(f, s, n, lds, lane); break;
+                default: r[0] = clover_wave<FT, 3>(f, s, n, lds, lane); break;
             }
         }
-        block_sum<3>(r, lds, 3);
-        if (tile < ntiles && threadIdx.x == 0 && threadIdx.y == 0) {
+        __syncthreads();
+        if (valid) clover_pair_terms(lds, lane, (int)threadIdx.y, r[1], r[2]);
+        block_sum<3>(r, red, 4);
+        if (threadIdx.x == 0 && threadIdx.y == 0) {
             partials[3 * tile + 0] = r[0];
             partials[3 * tile + 1] = r[1];
             partials[3 * tile + 2] = r[2];
         }
-        __syncthreads();  // lds is reused by the next iteration
+        __syncthreads();  // lds and red are reused by the next tile
     }
+}
+
+// RK3 stage 1 fused with the measurement of its input, for every member
+// (k_rk3_stage1_measure, flow.hip); members must use 64-consecutive-site tiles.
+template <prec P, int MU>
+__device__ inline double batch_stage1_measure_wave(const typename storage<P>::field& f,
+                                                   typename storage<P>::vec* __restrict__ out,
+                                                   typename storage<P>::avec* __restrict__ a,
+                                                   unsigned s, const nbr& n, double cz, bool& w,
+                                                   typename storage<P>::real* lds, unsigned lane) {
+    clover_parts<typename storage<P>::field, MU> h{f, s, n, f.link(MU, s), lds, lane, 0.0, {}};
+    stage_body<P, 1, MU>(f, out, a, s, n, cz, 0.0, w, h);
+    h.finish();
+    return h.plaq;
+}
+
+template <prec P>
+__global__ __launch_bounds__(256) void k_batch_stage1_measure(
+    const typename storage<P>::vec* __restrict__ Uin, typename storage<P>::vec* __restrict__ Uout,
+    typename storage<P>::avec* __restrict__ acc, const cfg_desc* __restrict__ cfgs,
+    const unsigned* __restrict__ tile_cfg, unsigned ntiles, int map_mode, unsigned map_win,
+    double cz, int* __restrict__ warn, double* __restrict__ partials) {
+    using FT = typename storage<P>::field;
+    __shared__ typename FT::real lds[kCloverSlots * 9 * 64];
+    __shared__ double red[4 * 3];
+    const unsigned nslots = grid_for_tiles(ntiles, map_mode, map_win);
+    const unsigned lane = threadIdx.x;
+    bool w = false;
+    for (unsigned v0 = blockIdx.x; v0 < nslots; v0 += gridDim.x) {  // uniform per workgroup
+        const unsigned tile = tile_of_block(v0, ntiles, map_mode, map_win);
+        if (tile >= ntiles) continue;
+        const cfg_desc& c = cfgs[tile_cfg[tile]];
+        const unsigned s = (tile - c.tile0) * kSitesPerBlock + lane;
+        const bool valid = s < c.g.V;
+        const FT f = storage<P>::view(Uin + c.u_off, c.g.V);
+        const nbr n = neighbours(c.g, valid ? s : 0u);
+        auto* out = Uout + c.u_off;
+        auto* a = acc + c.a_off;
+        double r[3] = {0.0, 0.0, 0.0};
+        if (valid) {
+            switch (threadIdx.y) {
+                case 0: r[0] = batch_stage1_measure_wave<P, 0>(f, out, a, s, n, cz, w, lds, lane); break;
+                case 1: r[0] = batch_stage1_measure_wave<P, 1>(f, out, a, s, n, cz, w, lds, lane); break;
+                case 2: r[0] = batch_stage1_measure_wave<P, 2>(f, out, a, s, n, cz, w, lds, lane); break;
+                default: r[0] = batch_stage1_measure_wave<P, 3>(f, out, a, s, n, cz, w, lds, lane); break;
+            }
+        }
+        __syncthreads();
+        if (valid) clover_pair_terms(lds, lane, (int)threadIdx.y, r[1], r[2]);
+        block_sum<3>(r, red, 4);
+        if (threadIdx.x == 0 && threadIdx.y == 0) {
+            partials[3 * tile + 0] = r[0];
+            partials[3 * tile + 1] = r[1];
+            partials[3 * tile + 2] = r[2];
+        }
+        __syncthreads();  // lds and red are reused by the next tile
+    }
+    if (w) atomicOr(warn, 1);
 }
 
 // One workgroup per configuration: ordered, compensated sum of its tiles.
@@ -164,7 +227,7 @@ hipError_t launch_batch_measure(prec p, const void* U, const cfg_desc* cfgs,
                                 const batch_launch& bl, double* partials, double* out,
                                 unsigned* slot_ctr, bool bump, hipStream_t st) {
     (void)hipGetLastError();  // drop a stale error of an earlier call
-    const dim3 block(kSitesPerBlock, 3);
+    const dim3 block(kSitesPerBlock, 4);
     if (p == prec::f64)
         k_batch_measure<prec::f64><<<bl.grid, block, 0, st>>>(
             static_cast<const double2*>(U), cfgs, tile_cfg, ntiles, bl.map_mode, bl.map_win,
@@ -173,6 +236,27 @@ hipError_t launch_batch_measure(prec p, const void* U, const cfg_desc* cfgs,
         k_batch_measure<prec::f32m><<<bl.grid, block, 0, st>>>(
             static_cast<const float4*>(U), cfgs, tile_cfg, ntiles, bl.map_mode, bl.map_win,
             partials);
+    k_batch_reduce<<<ncfg, 256, 0, st>>>(partials, cfgs, ncfg, out, slot_ctr);
+    if (bump && slot_ctr) k_bump<<<1, 1, 0, st>>>(slot_ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_stage1_measure(prec p, const void* Uin, void* Uout, void* acc,
+                                       const cfg_desc* cfgs, const unsigned* tile_cfg,
+                                       unsigned ntiles, int ncfg, const batch_launch& bl,
+                                       double cz, int* warn, double* partials, double* out,
+                                       unsigned* slot_ctr, bool bump, hipStream_t st) {
+    (void)hipGetLastError();  // drop a stale error of an earlier call
+    const dim3 block(kSitesPerBlock, 4);
+    if (p == prec::f64)
+        k_batch_stage1_measure<prec::f64><<<bl.grid, block, 0, st>>>(
+            static_cast<const double2*>(Uin), static_cast<double2*>(Uout),
+            static_cast<double2*>(acc), cfgs, tile_cfg, ntiles, bl.map_mode, bl.map_win, cz, warn,
+            partials);
+    else
+        k_batch_stage1_measure<prec::f32m><<<bl.grid, block, 0, st>>>(
+            static_cast<const float4*>(Uin), static_cast<float4*>(Uout), static_cast<float4*>(acc),
+            cfgs, tile_cfg, ntiles, bl.map_mode, bl.map_win, cz, warn, partials);
     k_batch_reduce<<<ncfg, 256, 0, st>>>(partials, cfgs, ncfg, out, slot_ctr);
     if (bump && slot_ctr) k_bump<<<1, 1, 0, st>>>(slot_ctr);
     return hipGetLastError();

@@ -42,6 +42,15 @@ hipError_t launch_batch_measure(prec p, const void* U, const cfg_desc* cfgs,
                                 const batch_launch& bl, double* partials, double* out,
                                 unsigned* slot_ctr, bool bump, hipStream_t st);
 
+// RK3 stage 1 of every member fused with the measurement of its input; the
+// measurement output is that of launch_batch_measure.  Every member must use
+// 64-consecutive-site stage tiles.
+hipError_t launch_batch_stage1_measure(prec p, const void* Uin, void* Uout, void* acc,
+                                       const cfg_desc* cfgs, const unsigned* tile_cfg,
+                                       unsigned ntiles, int ncfg, const batch_launch& bl,
+                                       double cz, int* warn, double* partials, double* out,
+                                       unsigned* slot_ctr, bool bump, hipStream_t st);
+
 // slot counter reset by a kernel (graph-capturable, no memset node)
 hipError_t launch_set_counter(unsigned* ctr, unsigned value, hipStream_t st);
 

@@ -24,6 +24,8 @@
 // (tile_of_block, lattice.hpp).
 #include "flow_body.hpp"
 #include "kernels.hpp"
+#include "measure_body.hpp"
+#include "reduce.hpp"
 
 namespace qcb {
 
@@ -78,6 +80,88 @@ hipError_t launch_rk3_stage(prec p, int stage, const void* Uin, void* Uout, void
     (void)hipGetLastError();  // drop a stale error of an earlier call
     return p == prec::f64 ? stage_launch<prec::f64>(stage, Uin, Uout, acc, g, cz, ca, warn, st)
                           : stage_launch<prec::f32m>(stage, Uin, Uout, acc, g, cz, ca, warn, st);
+}
+
+// ---- RK3 stage 1 fused with the measurement of its input -------------------
+// topo_charge_flowed measures the field and then flows it, so every
+// measurement but the last is followed by an RK3 stage 1 on the same field.
+// That stage already forms the upper and lower staples of every link, which
+// give three of the four leaves of each clover (clover_parts,
+// measure_body.hpp); this kernel does the stage and the measurement in one
+// pass, adding 8-10 SU(3) products per site and direction to the stage
+// instead of a separate measurement.  Its per-tile partials are those of
+// k_measure_clover (same per-site code, same workgroup reduction).  Stage
+// tiles must be 64 consecutive sites (the default shape).
+template <prec P, int MU>
+__device__ inline double stage1_measure_wave(const typename storage<P>::field& f,
+                                             typename storage<P>::vec* __restrict__ Uout,
+                                             typename storage<P>::avec* __restrict__ acc,
+                                             unsigned s, const nbr& n, double cz, bool& warn,
+                                             typename storage<P>::real* lds, unsigned lane) {
+    clover_parts<typename storage<P>::field, MU> h{f, s, n, f.link(MU, s), lds, lane, 0.0, {}};
+    stage_body<P, 1, MU>(f, Uout, acc, s, n, cz, 0.0, warn, h);
+    h.finish();
+    return h.plaq;
+}
+
+template <prec P>
+__global__ __launch_bounds__(256) void k_rk3_stage1_measure(
+    const typename storage<P>::vec* __restrict__ Uin, typename storage<P>::vec* __restrict__ Uout,
+    typename storage<P>::avec* __restrict__ acc, lattice g, double cz, int* __restrict__ warn,
+    double* __restrict__ partials) {
+    using FT = typename storage<P>::field;
+    static_assert(QCB_SU3_PRODUCTS && !FT::stream_rows,
+                  "the fused measurement takes its leaves from SU(3)-product staples");
+    __shared__ typename FT::real lds[kCloverSlots * 9 * 64];
+    __shared__ double red[4 * 3];
+    const unsigned ntiles = measure_blocks(g);
+    const unsigned tile =
+        tile_of_block(blockIdx.x, ntiles, g.map_mode, g.map_win, site_tiles_per_t(g));
+    if (tile >= ntiles) return;  // a whole padding workgroup: no barrier is skipped
+    const unsigned lane = threadIdx.x;
+    const unsigned s = tile * kSitesPerBlock + lane;
+    const bool valid = s < g.V;
+    const FT f = storage<P>::view(Uin, g.V);
+    const nbr n = neighbours(g, valid ? s : 0u);
+    double v[3] = {0.0, 0.0, 0.0};
+    bool w = false;
+    if (valid) {
+        switch (threadIdx.y) {
+            case 0: v[0] = stage1_measure_wave<P, 0>(f, Uout, acc, s, n, cz, w, lds, lane); break;
+            case 1: v[0] = stage1_measure_wave<P, 1>(f, Uout, acc, s, n, cz, w, lds, lane); break;
+            case 2: v[0] = stage1_measure_wave<P, 2>(f, Uout, acc, s, n, cz, w, lds, lane); break;
+            default: v[0] = stage1_measure_wave<P, 3>(f, Uout, acc, s, n, cz, w, lds, lane); break;
+        }
+    }
+    __syncthreads();
+    if (valid) clover_pair_terms(lds, lane, (int)threadIdx.y, v[1], v[2]);
+    block_sum<3>(v, red, 4);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        partials[3 * tile + 0] = v[0];
+        partials[3 * tile + 1] = v[1];
+        partials[3 * tile + 2] = v[2];
+    }
+    if (w) atomicOr(warn, 1);
+}
+
+hipError_t launch_rk3_stage1_measure(prec p, const void* Uin, void* Uout, void* acc,
+                                     const lattice& g, double cz, int* warn, double* partials,
+                                     double* out, hipStream_t st) {
+    (void)hipGetLastError();  // drop a stale error of an earlier call
+    if (g.lg_tile[0] >= 0) return hipErrorInvalidValue;  // box tiles: not fusable
+    const unsigned nblk = measure_blocks(g);
+    const dim3 grid(grid_for_tiles(nblk, g.map_mode, g.map_win, site_tiles_per_t(g)));
+    const dim3 block(kSitesPerBlock, 4);
+    if (p == prec::f64)
+        k_rk3_stage1_measure<prec::f64><<<grid, block, 0, st>>>(
+            static_cast<const double2*>(Uin), static_cast<double2*>(Uout),
+            static_cast<double2*>(acc), g, cz, warn, partials);
+    else
+        k_rk3_stage1_measure<prec::f32m><<<grid, block, 0, st>>>(
+            static_cast<const float4*>(Uin), static_cast<float4*>(Uout), static_cast<float4*>(acc),
+            g, cz, warn, partials);
+    const hipError_t e = hipGetLastError();
+    return e != hipSuccess ? e : launch_reduce_measure(partials, g, out, st);
 }
 
 // ---- unfused pieces (GradientFlow member functions) -------------------------

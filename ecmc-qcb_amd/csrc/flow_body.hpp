@@ -12,14 +12,44 @@ namespace qcb {
 #define QCB_SCHED_SPLIT 1
 #endif
 
+// Upper and lower staple of U_mu(x) in the plane (mu, nu), each in SU(3):
+//   upper = U_nu(x+mu) (U_nu(x) U_mu(x+nu))^dag
+//   lower = (U_mu(x-nu) U_nu(x+mu-nu))^dag U_nu(x-nu)
+// (the two terms of GaugeField::compute_staple, src/gauge/GaugeField.cpp:74-97)
+template <class F, int MU>
+__device__ inline mat3<typename F::real> upper_staple(const F& f, unsigned s, const nbr& n,
+                                                      int nu) {
+    const unsigned xmu = s + n.up[MU], xnu = s + n.up[nu];
+    return mul_nd_su3(f.link(nu, xmu), mul_su3(f.link(nu, s), f.link(MU, xnu)));
+}
+template <class F, int MU>
+__device__ inline mat3<typename F::real> lower_staple(const F& f, unsigned s, const nbr& n,
+                                                      int nu) {
+    const unsigned xmnu = s + n.dn[nu], xmunu = s + n.up[MU] + n.dn[nu];
+    return mul_dn_su3(mul_su3(f.link(MU, xmnu), f.link(nu, xmunu)), f.link(nu, xmnu));
+}
+
+// Receives the two staples of every plane as the staple sum forms them, the
+// upper one first (the fused measurement builds three clover leaves from
+// them); the default does nothing.
+struct no_staple_hook {
+    template <class M>
+    __device__ void upper(int, const M&) {}
+    template <class M>
+    __device__ void lower(int, const M&) {}
+};
+
 // Staple sum of U_mu(x):
 //   sum_{nu != mu} U_nu(x+mu) (U_nu(x) U_mu(x+nu))^dag
 //                + (U_mu(x-nu) U_nu(x+mu-nu))^dag U_nu(x-nu)
-// For FP64 one operand of every product is streamed from memory a row at a
-// time, so only the accumulator, one product and one full operand are live;
-// FP32 matrices are half the registers and are loaded whole.
-template <class F, int MU>
-__device__ inline mat3<typename F::real> staple_sum(const F& f, unsigned s, const nbr& n) {
+// With SU(3) products (the default) each term is upper_staple / lower_staple.
+// Without them, for FP64 one operand of every product is streamed from memory
+// a row at a time, so only the accumulator, one product and one full operand
+// are live; FP32 matrices are half the registers and are loaded whole.  The
+// hook is called only on the SU(3)-product path.
+template <class F, int MU, class Hook = no_staple_hook>
+__device__ inline mat3<typename F::real> staple_sum(const F& f, unsigned s, const nbr& n,
+                                                    Hook&& hook = Hook{}) {
     using T = typename F::real;
     mat3<T> st = zero3<T>();
     const unsigned xmu = s + n.up[MU];
@@ -75,11 +105,19 @@ __device__ inline mat3<typename F::real> staple_sum(const F& f, unsigned s, cons
             }
         } else if constexpr (QCB_SU3_PRODUCTS) {
             // every product of links is in SU(3): form two rows, rebuild the third
-            add_to(st, mul_nd_su3(f.link(nu, xmu), mul_su3(f.link(nu, s), f.link(MU, xnu))));
+            {
+                const mat3<T> wu = upper_staple<F, MU>(f, s, n, nu);
+                add_to(st, wu);
+                hook.upper(nu, wu);
+            }
 #if QCB_SCHED_SPLIT >= 2
             if constexpr (sizeof(T) == 8) __builtin_amdgcn_sched_barrier(0);
 #endif
-            add_to(st, mul_dn_su3(mul_su3(f.link(MU, xmnu), f.link(nu, xmunu)), f.link(nu, xmnu)));
+            {
+                const mat3<T> wd = lower_staple<F, MU>(f, s, n, nu);
+                add_to(st, wd);
+                hook.lower(nu, wd);
+            }
         } else {
             {
                 const mat3<T> a = mul(f.link(nu, s), f.link(MU, xnu));
@@ -102,23 +140,25 @@ __device__ inline mat3<typename F::real> staple_sum(const F& f, unsigned s, cons
     return st;
 }
 
-template <class F, int MU>
+template <class F, int MU, class Hook = no_staple_hook>
 __device__ inline alg8<typename F::real> force_at(const F& f, unsigned s, const nbr& n,
-                                                  mat3<typename F::real>& u) {
-    const mat3<typename F::real> st = staple_sum<F, MU>(f, s, n);
+                                                  mat3<typename F::real>& u,
+                                                  Hook&& hook = Hook{}) {
+    const mat3<typename F::real> st = staple_sum<F, MU>(f, s, n, hook);
     u = f.link(MU, s);
     return neg_proj_ta(mul(u, st));
 }
 
-template <prec P, int STAGE, int MU>
+template <prec P, int STAGE, int MU, class Hook = no_staple_hook>
 __device__ inline void stage_body(const typename storage<P>::field& f,
                                   typename storage<P>::vec* __restrict__ Uout,
                                   typename storage<P>::avec* __restrict__ acc, unsigned s,
-                                  const nbr& n, double cz, double ca, bool& warn) {
+                                  const nbr& n, double cz, double ca, bool& warn,
+                                  Hook&& hook = Hook{}) {
     using S = storage<P>;
     using T = typename S::real;
     mat3<T> u;
-    const alg8<T> z = force_at<typename S::field, MU>(f, s, n, u);
+    const alg8<T> z = force_at<typename S::field, MU>(f, s, n, u, hook);
     const T czr = (T)cz, car = (T)ca;
     alg8<T> x;
     if constexpr (STAGE == 1) {

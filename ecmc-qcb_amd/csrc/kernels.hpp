@@ -28,6 +28,14 @@ __host__ __device__ inline unsigned measure_blocks(const lattice& g) {
 hipError_t launch_rk3_stage(prec p, int stage, const void* Uin, void* Uout, void* acc,
                             const lattice& g, double cz, double ca, int* warn, hipStream_t st);
 
+// RK3 stage 1 fused with the clover measurement of Uin: the stage as above,
+// plus the measurement's per-tile partials and their reduction into
+// out[0..2] = {P, E, Q}, bit-identical to launch_measure(clover = true).
+// Needs the default stage tiles (64 consecutive sites).
+hipError_t launch_rk3_stage1_measure(prec p, const void* Uin, void* Uout, void* acc,
+                                     const lattice& g, double cz, int* warn, double* partials,
+                                     double* out, hipStream_t st);
+
 // Unfused pieces, one per GradientFlow member function.
 hipError_t launch_force(prec p, const void* U, void* F, const lattice& g, hipStream_t st);
 // which = 1: U <- exp(eps/4 F0) U ; 2: exp(8/9 eps F1 - 17/36 eps F0) U ;
@@ -44,6 +52,10 @@ hipError_t launch_replace_force0(prec p, void* F0, const void* F1, const lattice
 // computes the plaquette only (out[1], out[2] = 0).
 hipError_t launch_measure(prec p, const void* U, const lattice& g, double* partials, double* out,
                           bool clover, hipStream_t st);
+// The ordered reduction of launch_measure alone (partials of measure_blocks(g)
+// tiles -> out[0..2]).
+hipError_t launch_reduce_measure(const double* partials, const lattice& g, double* out,
+                                 hipStream_t st);
 // max over links of |U U^dag - 1|_max and of |det U - 1|, into out[0..1]
 // (partials sized 2*measure_blocks)
 hipError_t launch_unitarity(prec p, const void* U, const lattice& g, double* partials,

@@ -9,19 +9,59 @@
 // The FP32-mixed path forms the clovers in FP32 and accumulates every
 // per-site contribution in FP64; the final reduction is Neumaier-compensated.
 //
-// Work mapping: a workgroup is 64 consecutive sites x 3 plane pairs.  Pair 0
-// handles planes (0,1),(2,3), pair 1 (0,2),(1,3), pair 2 (0,3),(1,2), so each
-// thread owns one term of q = Re tr F01F23 - Re tr F02F13 + Re tr F03F12 and
-// two of the six energy terms and plaquettes.
+// Work mapping (clover): a workgroup is 64 consecutive sites x 4 directions.
+// Wavefront mu forms, from the staples of U_mu(x), the clover leaves it owns in
+// its three planes (clover_parts, measure_body.hpp) and leaves their
+// anti-hermitian parts in LDS; after one barrier wavefronts 0-2 each take one
+// plane pair, (0,1)(2,3), (0,2)(1,3) or (0,3)(1,2), and form one term of
+// q = Re tr F01F23 - Re tr F02F13 + Re tr F03F12 and two of the six energy
+// terms.  The fused RK3 stage 1 (flow.hip) runs the same per-site code.
+//
+// The plaquette-only measurement keeps one wavefront per plane pair.
 #include "kernels.hpp"
 #include "measure_body.hpp"
 #include "reduce.hpp"
 
 namespace qcb {
 
-template <prec P, bool CLOVER>
-__global__ __launch_bounds__(192) void k_measure(const typename storage<P>::vec* __restrict__ U,
-                                                 lattice g, double* __restrict__ partials) {
+template <prec P>
+__global__ __launch_bounds__(256) void k_measure_clover(const typename storage<P>::vec* __restrict__ U,
+                                                        lattice g, double* __restrict__ partials) {
+    using FT = typename storage<P>::field;
+    using T = typename FT::real;
+    __shared__ T lds[kCloverSlots * 9 * 64];
+    __shared__ double red[4 * 3];
+    const unsigned ntiles = measure_blocks(g);
+    const unsigned tile =
+        tile_of_block(blockIdx.x, ntiles, g.map_mode, g.map_win, site_tiles_per_t(g));
+    if (tile >= ntiles) return;  // a whole padding workgroup: no barrier is skipped
+    const unsigned lane = threadIdx.x;
+    const unsigned s = tile * kSitesPerBlock + lane;
+    const bool valid = s < g.V;
+    double v[3] = {0.0, 0.0, 0.0};
+    const FT f = storage<P>::view(U, g.V);
+    const nbr n = neighbours(g, valid ? s : 0u);
+    if (valid) {
+        switch (threadIdx.y) {
+            case 0: v[0] = clover_wave<FT, 0>(f, s, n, lds, lane); break;
+            case 1: v[0] = clover_wave<FT, 1>(f, s, n, lds, lane); break;
+            case 2: v[0] = clover_wave<FT, 2>(f, s, n, lds, lane); break;
+            default: v[0] = clover_wave<FT, 3>(f, s, n, lds, lane); break;
+        }
+    }
+    __syncthreads();
+    if (valid) clover_pair_terms(lds, lane, (int)threadIdx.y, v[1], v[2]);
+    block_sum<3>(v, red, 4);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        partials[3 * tile + 0] = v[0];
+        partials[3 * tile + 1] = v[1];
+        partials[3 * tile + 2] = v[2];
+    }
+}
+
+template <prec P>
+__global__ __launch_bounds__(192) void k_plaquette(const typename storage<P>::vec* __restrict__ U,
+                                                   lattice g, double* __restrict__ partials) {
     using FT = typename storage<P>::field;
     __shared__ double lds[3 * 3];
     const unsigned ntiles = measure_blocks(g);
@@ -34,9 +74,18 @@ __global__ __launch_bounds__(192) void k_measure(const typename storage<P>::vec*
         const nbr n = neighbours(g, s);
         const FT f = storage<P>::view(U, g.V);
         switch (threadIdx.y) {
-            case 0: pair_body<FT, 0, 1, 2, 3, +1, CLOVER>(f, s, n, v[0], v[1], v[2]); break;
-            case 1: pair_body<FT, 0, 2, 1, 3, -1, CLOVER>(f, s, n, v[0], v[1], v[2]); break;
-            default: pair_body<FT, 0, 3, 1, 2, +1, CLOVER>(f, s, n, v[0], v[1], v[2]); break;
+            case 0:
+                plaq_only<FT, 0, 1>(f, s, n, v[0]);
+                plaq_only<FT, 2, 3>(f, s, n, v[0]);
+                break;
+            case 1:
+                plaq_only<FT, 0, 2>(f, s, n, v[0]);
+                plaq_only<FT, 1, 3>(f, s, n, v[0]);
+                break;
+            default:
+                plaq_only<FT, 0, 3>(f, s, n, v[0]);
+                plaq_only<FT, 1, 2>(f, s, n, v[0]);
+                break;
         }
     }
     block_sum<3>(v, lds, 3);
@@ -72,13 +121,19 @@ static hipError_t measure_launch(const void* U, const lattice& g, double* partia
                                  bool clover, hipStream_t st) {
     const unsigned nblk = measure_blocks(g);
     const unsigned grid = grid_for_tiles(nblk, g.map_mode, g.map_win, site_tiles_per_t(g));
-    const dim3 block(kSitesPerBlock, 3);
     const auto* u = static_cast<const typename storage<P>::vec*>(U);
     if (clover)
-        k_measure<P, true><<<grid, block, 0, st>>>(u, g, partials);
+        k_measure_clover<P><<<grid, dim3(kSitesPerBlock, 4), 0, st>>>(u, g, partials);
     else
-        k_measure<P, false><<<grid, block, 0, st>>>(u, g, partials);
+        k_plaquette<P><<<grid, dim3(kSitesPerBlock, 3), 0, st>>>(u, g, partials);
     k_reduce_measure<<<1, 256, 0, st>>>(partials, nblk, 1.0 / (double)g.V, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_measure(const double* partials, const lattice& g, double* out,
+                                 hipStream_t st) {
+    (void)hipGetLastError();  // drop a stale error of an earlier call
+    k_reduce_measure<<<1, 256, 0, st>>>(partials, measure_blocks(g), 1.0 / (double)g.V, out);
     return hipGetLastError();
 }
 

@@ -1,7 +1,7 @@
 // measure_body.hpp -- per-site device code of the clover / plaquette / Q / E
 // measurement, shared by measure.hip and batch.hip.
 #pragma once
-#include "fields.hpp"
+#include "flow_body.hpp"
 
 namespace qcb {
 
@@ -13,196 +13,15 @@ struct fclov {
     cpx<T> o[3];  // (0,1), (0,2), (1,2)
 };
 
-// Products whose right operand is streamed from memory a row at a time:
-//   R  = A B        (rows k of B)        R += A B      (rows k of B)
-//   R  = A B^dag    (rows j of B)        R += A B^dag  (rows j of B)
-//   R  = A^dag B    (rows k of A and B)  R  = M B      (rows i of M, B whole)
-template <class F>
-__device__ inline void acc_mul_s(mat3<typename F::real>& R, const mat3<typename F::real>& A,
-                                 const F& f, int mu, unsigned s) {
-    using T = typename F::real;
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        cpx<T> b[3];
-        f.row(mu, s, k, b);
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++) R.e[i][j] = madd(R.e[i][j], A.e[i][k], b[j]);
-    }
-}
-template <class F>
-__device__ inline void acc_mul_nd_s(mat3<typename F::real>& R, const mat3<typename F::real>& A,
-                                    const F& f, int mu, unsigned s) {
-    using T = typename F::real;
-#pragma unroll
-    for (int j = 0; j < 3; j++) {
-        cpx<T> b[3];
-        f.row(mu, s, j, b);
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-            R.e[i][j] = madd_c(madd_c(madd_c(R.e[i][j], A.e[i][0], b[0]), A.e[i][1], b[1]), A.e[i][2],
-                               b[2]);
-    }
-}
-template <class F>
-__device__ inline mat3<typename F::real> mul_rows_s(const F& f, int mu, unsigned s,
-                                                    const mat3<typename F::real>& B) {
-    using T = typename F::real;
-    mat3<T> R;
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        cpx<T> a[3];
-        f.row(mu, s, i, a);
-#pragma unroll
-        for (int j = 0; j < 3; j++)
-            R.e[i][j] = madd(madd(a[0] * B.e[0][j], a[1], B.e[1][j]), a[2], B.e[2][j]);
-    }
-    return R;
-}
-template <class F>
-__device__ inline mat3<typename F::real> mul_dn_s(const F& f, int mua, unsigned sa, int mub,
-                                                  unsigned sb) {
-    using T = typename F::real;
-    mat3<T> R = zero3<T>();
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        cpx<T> a[3], b[3];
-        f.row(mua, sa, k, a);
-        f.row(mub, sb, k, b);
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++) R.e[i][j] = madd_cl(R.e[i][j], a[i], b[j]);
-    }
-    return R;
-}
-
-// The four clover leaves with streamed operands (FP64 storage): at most the
-// running sum, two products and one whole link are live.
-template <class F, int MU, int NU>
-__device__ inline mat3<typename F::real> clover_leaves_streamed(
-    const F& f, unsigned s, unsigned xpmu, unsigned xpnu, unsigned xmmu, unsigned xmnu,
-    unsigned xmmupnu, unsigned xmmumnu, unsigned xpmumnu, double& plaq) {
-    using T = typename F::real;
-    mat3<T> C = zero3<T>();
-    {  // U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
-        const mat3<T> a = mul_rows_s(f, MU, s, f.link(NU, xpmu));
-        mat3<T> b = zero3<T>();
-        acc_mul_nd_s(b, a, f, MU, xpnu);
-        acc_mul_nd_s(C, b, f, NU, s);
-        plaq += (double)(C.e[0][0].r + C.e[1][1].r + C.e[2][2].r);
-    }
-    {  // U_nu(x) U_mu(x-mu+nu)^dag U_nu(x-mu)^dag U_mu(x-mu)
-        mat3<T> a = zero3<T>();
-        acc_mul_nd_s(a, f.link(NU, s), f, MU, xmmupnu);
-        mat3<T> b = zero3<T>();
-        acc_mul_nd_s(b, a, f, NU, xmmu);
-        acc_mul_s(C, b, f, MU, xmmu);
-    }
-    {  // U_mu(x-mu)^dag U_nu(x-mu-nu)^dag U_mu(x-mu-nu) U_nu(x-nu)
-        //  = (U_nu(x-mu-nu) U_mu(x-mu))^dag U_mu(x-mu-nu) U_nu(x-nu)
-        const mat3<T> a = mul_rows_s(f, NU, xmmumnu, f.link(MU, xmmu));
-        mat3<T> d = zero3<T>();
-#pragma unroll
-        for (int k = 0; k < 3; k++) {  // d = a^dag U_mu(x-mu-nu)
-            cpx<T> b[3];
-            f.row(MU, xmmumnu, k, b);
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-                for (int j = 0; j < 3; j++) d.e[i][j] = madd_cl(d.e[i][j], a.e[k][i], b[j]);
-        }
-        acc_mul_s(C, d, f, NU, xmnu);
-    }
-    {  // U_nu(x-nu)^dag U_mu(x-nu) U_nu(x+mu-nu) U_mu(x)^dag
-        const mat3<T> a = mul_dn_s(f, NU, xmnu, MU, xmnu);
-        mat3<T> b = zero3<T>();
-        acc_mul_s(b, a, f, NU, xpmumnu);
-        acc_mul_nd_s(C, b, f, MU, s);
-    }
-    return C;
-}
-
-// Sum of the four leaves of plane (MU,NU) at x, all anticlockwise, in the
-// order and association of clover_site (observables_mpi.cpp:78-86).  Adds
-// Re tr(leaf 0) to plaq.
-template <class F, int MU, int NU>
-__device__ inline fclov<typename F::real> clover(const F& f, unsigned s, const nbr& n,
-                                                 double& plaq) {
-    using T = typename F::real;
-    const unsigned xpmu = s + n.up[MU], xpnu = s + n.up[NU];
-    const unsigned xmmu = s + n.dn[MU], xmnu = s + n.dn[NU];
-    const unsigned xmmupnu = xmmu + n.up[NU], xmmumnu = xmmu + n.dn[NU];
-    const unsigned xpmumnu = xpmu + n.dn[NU];
-    mat3<T> C;
-    if constexpr (F::stream_rows) {
-        C = clover_leaves_streamed<F, MU, NU>(f, s, xpmu, xpnu, xmmu, xmnu, xmmupnu, xmmumnu,
-                                              xpmumnu, plaq);
-    } else if constexpr (QCB_SU3_PRODUCTS) {
-        // every partial product of a leaf is in SU(3): two rows + cross product
-        {
-            const mat3<T> b = mul_nd_su3(mul_su3(f.link(MU, s), f.link(NU, xpmu)), f.link(MU, xpnu));
-            C = mul_nd_su3(b, f.link(NU, s));
-            plaq += (double)(C.e[0][0].r + C.e[1][1].r + C.e[2][2].r);
-        }
-        {
-            const mat3<T> b =
-                mul_nd_su3(mul_nd_su3(f.link(NU, s), f.link(MU, xmmupnu)), f.link(NU, xmmu));
-            add_to(C, mul_su3(b, f.link(MU, xmmu)));
-        }
-        add_to(C, mul_dn_su3(mul_su3(f.link(NU, xmmumnu), f.link(MU, xmmu)),
-                             mul_su3(f.link(MU, xmmumnu), f.link(NU, xmnu))));
-        {
-            const mat3<T> b = mul_su3(mul_dn_su3(f.link(NU, xmnu), f.link(MU, xmnu)),
-                                      f.link(NU, xpmumnu));
-            add_to(C, mul_nd_su3(b, f.link(MU, s)));
-        }
-    } else {
-    {  // U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
-        const mat3<T> a = mul(f.link(MU, s), f.link(NU, xpmu));
-        const mat3<T> b = mul_nd(a, f.link(MU, xpnu));
-        C = mul_nd(b, f.link(NU, s));
-        plaq += (double)(C.e[0][0].r + C.e[1][1].r + C.e[2][2].r);
-    }
-    {  // U_nu(x) U_mu(x-mu+nu)^dag U_nu(x-mu)^dag U_mu(x-mu)
-        const mat3<T> a = mul_nd(f.link(NU, s), f.link(MU, xmmupnu));
-        const mat3<T> b = mul_nd(a, f.link(NU, xmmu));
-        acc_mul(C, b, f.link(MU, xmmu));
-    }
-    {  // U_mu(x-mu)^dag U_nu(x-mu-nu)^dag U_mu(x-mu-nu) U_nu(x-nu)
-        const mat3<T> a = mul(f.link(NU, xmmumnu), f.link(MU, xmmu));
-        const mat3<T> b = mul(f.link(MU, xmmumnu), f.link(NU, xmnu));
-        acc_mul_dn(C, a, b);
-    }
-    {  // U_nu(x-nu)^dag U_mu(x-nu) U_nu(x+mu-nu) U_mu(x)^dag
-        const mat3<T> a = mul_dn(f.link(NU, xmnu), f.link(MU, xmnu));
-        const mat3<T> b = mul(a, f.link(NU, xpmumnu));
-        acc_mul_nd(C, b, f.link(MU, s));
-    }
-    }
-    fclov<T> Fc;
-    const T e8 = (T)0.125;
-#pragma unroll
-    for (int k = 0; k < 3; k++) Fc.d[k] = (C.e[k][k].i + C.e[k][k].i) * e8;
-    Fc.o[0] = e8 * (C.e[0][1] - conj(C.e[1][0]));
-    Fc.o[1] = e8 * (C.e[0][2] - conj(C.e[2][0]));
-    Fc.o[2] = e8 * (C.e[1][2] - conj(C.e[2][1]));
-    return Fc;
-}
-
+// Re tr of the plaquette U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
+// (mean_plaquette_local, observables_mpi.cpp:11-35), for the plaquette-only
+// measurement.
 template <class F, int MU, int NU>
 __device__ inline void plaq_only(const F& f, unsigned s, const nbr& n, double& plaq) {
     using T = typename F::real;
     const unsigned xpmu = s + n.up[MU], xpnu = s + n.up[NU];
-    if constexpr (QCB_SU3_PRODUCTS && !F::stream_rows) {
-        const mat3<T> b = mul_nd_su3(mul_su3(f.link(MU, s), f.link(NU, xpmu)), f.link(MU, xpnu));
-        plaq += (double)re_tr_nd(b, f.link(NU, s));
-    } else {
-        const mat3<T> a = mul(f.link(MU, s), f.link(NU, xpmu));
-        const mat3<T> b = mul_nd(a, f.link(MU, xpnu));
-        plaq += (double)re_tr_nd(b, f.link(NU, s));
-    }
+    const mat3<T> b = mul_nd_su3(mul_su3(f.link(MU, s), f.link(NU, xpmu)), f.link(MU, xpnu));
+    plaq += (double)re_tr_nd(b, f.link(NU, s));
 }
 
 // |F|_F^2 for anti-hermitian F
@@ -221,18 +40,171 @@ __device__ inline T re_tr_ab(const fclov<T>& A, const fclov<T>& B) {
     return -(s + T(2) * o);
 }
 
-template <class F, int M1, int N1, int M2, int N2, int SIGN, bool CLOVER>
-__device__ inline void pair_body(const F& f, unsigned s, const nbr& n, double& vp, double& ve,
-                                 double& vq) {
-    if constexpr (CLOVER) {
-        const auto A = clover<F, M1, N1>(f, s, n, vp);
-        const auto B = clover<F, M2, N2>(f, s, n, vp);
-        ve += 0.5 * ((double)fnorm2(A) + (double)fnorm2(B));
-        vq += SIGN * (double)re_tr_ab(A, B);
-    } else {
-        plaq_only<F, M1, N1>(f, s, n, vp);
-        plaq_only<F, M2, N2>(f, s, n, vp);
+// ---- clover leaves from staples ----------------------------------------------
+// Of the four leaves of plane (a, b) at x (clover_site, observables_mpi.cpp:
+// 66-90), three are one SU(3) product away from the staples that the flow
+// forms for the links at x:
+//   leaf 0 = U_a(x) upper(a,b)            = (U_b(x) upper(b,a))^dag
+//   leaf 1 = U_b(x) lower(b,a)
+//   leaf 3 = (U_a(x) lower(a,b))^dag
+//   leaf 2 = (U_b(x-a-b) U_a(x-a))^dag U_a(x-a-b) U_b(x-b)   (formed directly)
+// where upper(mu,nu) / lower(mu,nu) are the staples of U_mu(x) in the plane
+// (flow_body.hpp).  F = (C - C^dag)/8 is linear in the leaves, so each
+// wavefront (direction mu) adds up the anti-hermitian parts of the leaves it
+// forms for each of its three planes and leaves them in LDS; the two
+// wavefronts of a plane then meet there.  The same code runs inside the fused
+// RK3 stage 1 (where the staples are already formed) and in the standalone
+// measurement, so both give bit-identical results.
+
+// Plane (a, b), a < b: (0,1) 0, (0,2) 1, (0,3) 2, (1,2) 3, (1,3) 4, (2,3) 5.
+__host__ __device__ constexpr int plane_of(int a, int b) { return a == 0 ? b - 1 : a + b; }
+// Wavefront that forms leaf 0 and leaf 2 of each plane.  Every wavefront forms
+// one leaf (1 or 3) per plane from its lower staples; with these owners the
+// four wavefronts form 8, 8, 10 and 10 SU(3) products per site.
+__host__ __device__ constexpr int leaf0_wave(int p) {
+    return p == 2 ? 3 : (p == 3 || p == 4) ? 1 : p == 5 ? 2 : 0;
+}
+__host__ __device__ constexpr int leaf2_wave(int p) {
+    return p == 0 ? 1 : (p == 1 || p == 3) ? 2 : p == 2 ? 0 : 3;
+}
+// LDS slot of wavefront mu's part of the plane (mu, nu): three per wavefront.
+__host__ __device__ constexpr int clover_slot(int mu, int nu) { return 3 * mu + (nu < mu ? nu : nu - 1); }
+constexpr int kCloverSlots = 12;
+
+// F += sign (M - M^dag)/8
+template <typename T>
+__device__ inline void add_ah8(fclov<T>& F, const mat3<T>& M, T sign) {
+    const T e = sign * (T)0.125;
+#pragma unroll
+    for (int k = 0; k < 3; k++) F.d[k] += e * (M.e[k][k].i + M.e[k][k].i);
+    F.o[0] = F.o[0] + e * (M.e[0][1] - conj(M.e[1][0]));
+    F.o[1] = F.o[1] + e * (M.e[0][2] - conj(M.e[2][0]));
+    F.o[2] = F.o[2] + e * (M.e[1][2] - conj(M.e[2][1]));
+}
+
+template <class F>
+__device__ inline mat3<typename F::real> clover_leaf2(const F& f, unsigned s, const nbr& n, int a,
+                                                      int b) {
+    const unsigned xma = s + n.dn[a], xmb = s + n.dn[b], y = xma + n.dn[b];
+    return mul_dn_su3(mul_su3(f.link(b, y), f.link(a, xma)), mul_su3(f.link(a, y), f.link(b, xmb)));
+}
+
+// Staple hook of wavefront MU.  For each plane (MU, nu) it forms the leaves
+// 0 (if it owns it) and 1 or 3 from the staples as they arrive, adds their
+// anti-hermitian parts and writes them to slot clover_slot(MU, nu) of lds
+// (layout [slot][9 reals][64 lanes]); finish() then adds the leaves 2 it owns,
+// formed from their own links, once the staples' registers are free.  The
+// real traces of the leaf-0 products go to plaq.
+template <class F, int MU>
+struct clover_parts {
+    using T = typename F::real;
+    const F& f;
+    unsigned s;
+    const nbr& n;
+    mat3<T> u;  // U_mu(x); rows 0-1 are read
+    T* lds;
+    unsigned lane;
+    double plaq;
+    fclov<T> c;  // the current plane's part
+
+    __device__ static constexpr int plane(int nu) {
+        return plane_of(MU < nu ? MU : nu, MU < nu ? nu : MU);
     }
+    __device__ T* slot(int nu) const { return lds + (size_t)clover_slot(MU, nu) * 9 * 64 + lane; }
+
+    __device__ void upper(int nu, const mat3<T>& wu) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            c.d[k] = T(0);
+            c.o[k] = {T(0), T(0)};
+        }
+        if (leaf0_wave(plane(nu)) == MU) {
+            const mat3<T> m = mul_su3(u, wu);  // leaf 0 (MU < nu) or its adjoint
+            add_ah8(c, m, MU < nu ? T(1) : T(-1));
+            plaq += (double)(m.e[0][0].r + m.e[1][1].r + m.e[2][2].r);
+        }
+    }
+    __device__ void lower(int nu, const mat3<T>& wd) {
+        // leaf 3 = (u lower)^dag when MU < nu, leaf 1 = u lower when MU > nu
+        add_ah8(c, mul_su3(u, wd), MU < nu ? T(-1) : T(1));
+        T* q = slot(nu);
+#pragma unroll
+        for (int k = 0; k < 3; k++) q[k * 64] = c.d[k];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            q[(3 + 2 * k) * 64] = c.o[k].r;
+            q[(4 + 2 * k) * 64] = c.o[k].i;
+        }
+    }
+    __device__ void finish() {
+#pragma unroll
+        for (int nu = 0; nu < 4; nu++) {
+            if (nu == MU || leaf2_wave(plane(nu)) != MU) continue;
+            fclov<T> l;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                l.d[k] = T(0);
+                l.o[k] = {T(0), T(0)};
+            }
+            add_ah8(l, clover_leaf2(f, s, n, MU < nu ? MU : nu, MU < nu ? nu : MU), T(1));
+            T* q = slot(nu);
+#pragma unroll
+            for (int k = 0; k < 3; k++) q[k * 64] += l.d[k];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                q[(3 + 2 * k) * 64] += l.o[k].r;
+                q[(4 + 2 * k) * 64] += l.o[k].i;
+            }
+        }
+    }
+};
+
+// Standalone measurement work of wavefront MU: the staples its leaves need.
+template <class F, int MU>
+__device__ inline double clover_wave(const F& f, unsigned s, const nbr& n,
+                                     typename F::real* lds, unsigned lane) {
+    clover_parts<F, MU> h{f, s, n, f.link(MU, s), lds, lane, 0.0, {}};
+#pragma unroll
+    for (int nu = 0; nu < 4; nu++) {
+        if (nu == MU) continue;
+        if (leaf0_wave(h.plane(nu)) == MU)
+            h.upper(nu, upper_staple<F, MU>(f, s, n, nu));
+        else
+            h.upper(nu, mat3<typename F::real>{});  // clears the plane's part only
+        h.lower(nu, lower_staple<F, MU>(f, s, n, nu));
+    }
+    h.finish();
+    return h.plaq;
+}
+
+// F of plane (a, b) from the two wavefronts' parts, wavefront a's first.
+template <typename T>
+__device__ inline fclov<T> clover_from_lds(const T* lds, unsigned lane, int a, int b) {
+    const T* pa = lds + (size_t)clover_slot(a, b) * 9 * 64 + lane;
+    const T* pb = lds + (size_t)clover_slot(b, a) * 9 * 64 + lane;
+    fclov<T> F;
+#pragma unroll
+    for (int k = 0; k < 3; k++) F.d[k] = pa[k * 64] + pb[k * 64];
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+        F.o[k] = {pa[(3 + 2 * k) * 64] + pb[(3 + 2 * k) * 64],
+                  pa[(4 + 2 * k) * 64] + pb[(4 + 2 * k) * 64]};
+    return F;
+}
+
+// After the workgroup barrier: wavefront w < 3 takes the plane pair
+// {(0,1),(2,3)}, {(0,2),(1,3)} or {(0,3),(1,2)} and adds its E and Q terms
+// (local_q_e_clover, observables_mpi.cpp:93-112).
+template <typename T>
+__device__ inline void clover_pair_terms(const T* lds, unsigned lane, int w, double& ve,
+                                         double& vq) {
+    if (w >= 3) return;
+    const int b = w + 1;                  // partner of 0
+    const int c = w == 0 ? 2 : 1, d = w == 2 ? 2 : 3;
+    const fclov<T> A = clover_from_lds(lds, lane, 0, b);
+    const fclov<T> B = clover_from_lds(lds, lane, c, d);
+    ve += 0.5 * ((double)fnorm2(A) + (double)fnorm2(B));
+    vq += (w == 1 ? -1.0 : 1.0) * (double)re_tr_ab(A, B);
 }
 
 }  // namespace qcb

@@ -49,8 +49,9 @@ struct qcbf_flow {
     bool timing = false;
     std::vector<qcbf_event> events;
     size_t ev_used = 0;
-    double tot_ms[QCBF_K_COUNT] = {0, 0, 0, 0};
-    long cnt[QCBF_K_COUNT] = {0, 0, 0, 0};
+    double tot_ms[QCBF_K_COUNT] = {};
+    long cnt[QCBF_K_COUNT] = {};
+    bool fuse_measure = true;         // flow_measure: measurement inside the next stage 1
 };
 
 // Sites per host <-> HBM transfer chunk (37.7 MB of FP64 links).
@@ -452,15 +453,24 @@ extern "C" int qcbf_save_ildg(qcbf_flow* h, const char* path) {
 }
 
 // ---- flow ------------------------------------------------------------------------
-static int rk3_once(qcbf_flow* h, double eps) {
+// One RK3 step.  With measure_out set, stage 1 also measures its input field
+// into measure_out[0..2] (the fused kernel).
+static int rk3_once(qcbf_flow* h, double eps, double* measure_out = nullptr) {
     const double cz[3] = {0.25 * eps, (8.0 / 9.0) * eps, 0.75 * eps};
     const double ca[3] = {0.0, (17.0 / 36.0) * eps, eps};
     for (int stage = 1; stage <= 3; stage++) {
         qcbf_event* ev;
-        int rc = timing_begin(h, QCBF_K_STAGE1 + stage - 1, &ev);
+        const bool fused = stage == 1 && measure_out;
+        int rc = timing_begin(h, fused ? QCBF_K_STAGE1_MEASURE : QCBF_K_STAGE1 + stage - 1, &ev);
         if (rc) return rc;
-        HIPCHK(h, qcb::launch_rk3_stage(h->prec, stage, h->U[h->cur], h->U[h->cur ^ 1], h->acc,
-                                        h->g, cz[stage - 1], ca[stage - 1], h->dwarn, h->stream));
+        if (fused)
+            HIPCHK(h, qcb::launch_rk3_stage1_measure(h->prec, h->U[h->cur], h->U[h->cur ^ 1],
+                                                     h->acc, h->g, cz[0], h->dwarn, h->partials,
+                                                     measure_out, h->stream));
+        else
+            HIPCHK(h, qcb::launch_rk3_stage(h->prec, stage, h->U[h->cur], h->U[h->cur ^ 1],
+                                            h->acc, h->g, cz[stage - 1], ca[stage - 1], h->dwarn,
+                                            h->stream));
         rc = timing_end(h, ev);
         if (rc) return rc;
         h->cur ^= 1;
@@ -595,13 +605,20 @@ extern "C" int qcbf_flow_measure(qcbf_flow* h, double eps, int n_steps_gf, int n
     if (rc) return rc;
     std::vector<double> tvals(n_steps_gf);
     double t = 0.0;
+    // Every measurement but the last is followed by an RK3 step on the same
+    // field: its stage 1 takes the measurement along (same results).
+    const bool fuse = h->fuse_measure && h->g.lg_tile[0] < 0 && n_rk_steps > 0;
     for (int n = 0; n < n_steps_gf; n++) {
         tvals[n] = t;
-        rc = measure_into(h, h->dres + 3 * (size_t)n, true);
-        if (rc) return rc;
-        if (n != n_steps_gf - 1) {
+        double* out = h->dres + 3 * (size_t)n;
+        const bool last = n == n_steps_gf - 1;
+        if (!(fuse && !last)) {
+            rc = measure_into(h, out, true);
+            if (rc) return rc;
+        }
+        if (!last) {
             for (int k = 0; k < n_rk_steps; k++) {
-                rc = rk3_once(h, eps);
+                rc = rk3_once(h, eps, fuse && k == 0 ? out : nullptr);
                 if (rc) return rc;
                 t += eps;
             }
@@ -693,6 +710,10 @@ extern "C" int qcbf_tune(qcbf_flow* h, int key, long value) {
                 return fail(h, QCBF_ERR_ARG, "map window %ld not in [1, 2^24]", value);
             h->g.map_win = (unsigned)value;
             return QCBF_OK;
+        case QCBF_TUNE_FUSED_MEASURE:
+            if (value != 0 && value != 1) return fail(h, QCBF_ERR_ARG, "fused measure %ld not 0 or 1", value);
+            h->fuse_measure = value == 1;
+            return QCBF_OK;
         case QCBF_TUNE_TILE_SHAPE: {
             if (value == -1) {
                 qcb::set_tile_shape(h->g, nullptr);
@@ -722,6 +743,7 @@ extern "C" int qcbf_get_tune(const qcbf_flow* h, int key, long* value) {
         case QCBF_TUNE_MAP_MODE: *value = h->g.map_mode; return QCBF_OK;
         case QCBF_TUNE_MAP_WIN: *value = h->g.map_win; return QCBF_OK;
         case QCBF_TUNE_TILE_SHAPE: *value = tile_code(h->g); return QCBF_OK;
+        case QCBF_TUNE_FUSED_MEASURE: *value = h->fuse_measure ? 1 : 0; return QCBF_OK;
         default: return fail(nullptr, QCBF_ERR_ARG, "unknown tuning key %d", key);
     }
 }

@@ -49,6 +49,7 @@ struct qcbf_batch {
     long tile_shape = -1;
     int cus = 256;
     bool use_graph = false;
+    bool fuse_measure = true;  // flow_measure: measurement inside the next stage 1
     // graph of one (measure, RK3 step) pair, keyed by (cur, eps)
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     double geps[2] = {0.0, 0.0};
@@ -343,17 +344,31 @@ extern "C" int qcbf_batch_copy_from(qcbf_batch* b, int c, qcbf_flow* src) {
 }
 
 // ---- flow -----------------------------------------------------------------------
-static int enqueue_step(qcbf_batch* b, double eps, int cur) {
+// One RK3 step of every member.  With out set, stage 1 also measures its input
+// into result slot *slot (bumped afterwards when bump is set), as
+// enqueue_measure would.
+static int enqueue_step(qcbf_batch* b, double eps, int cur, double* out = nullptr,
+                        unsigned* slot = nullptr, bool bump = false) {
     const double cz[3] = {0.25 * eps, (8.0 / 9.0) * eps, 0.75 * eps};
     const double ca[3] = {0.0, (17.0 / 36.0) * eps, eps};
     for (int stage = 1; stage <= 3; stage++) {
-        BCHK(b, qcb::launch_batch_stage(b->prec, stage, b->U[cur], b->U[cur ^ 1], b->acc, b->d_cfgs,
-                                        b->d_tile_cfg, b->ntiles, b->bl, cz[stage - 1],
-                                        ca[stage - 1], b->dwarn, b->stream));
+        if (stage == 1 && out)
+            BCHK(b, qcb::launch_batch_stage1_measure(b->prec, b->U[cur], b->U[cur ^ 1], b->acc,
+                                                     b->d_cfgs, b->d_tile_cfg, b->ntiles, b->n,
+                                                     b->bl, cz[0], b->dwarn, b->partials, out,
+                                                     slot, bump, b->stream));
+        else
+            BCHK(b, qcb::launch_batch_stage(b->prec, stage, b->U[cur], b->U[cur ^ 1], b->acc,
+                                            b->d_cfgs, b->d_tile_cfg, b->ntiles, b->bl,
+                                            cz[stage - 1], ca[stage - 1], b->dwarn, b->stream));
         cur ^= 1;
     }
     return QCBF_OK;
 }
+
+// The measurement can ride on the next stage 1 when every member uses
+// 64-consecutive-site tiles.
+static bool fusable(const qcbf_batch* b) { return b->fuse_measure && b->tile_shape == -1; }
 
 static int enqueue_measure(qcbf_batch* b, double* out, unsigned* slot, bool bump) {
     BCHK(b, qcb::launch_batch_measure(b->prec, b->U[b->cur], b->d_cfgs, b->d_tile_cfg, b->ntiles,
@@ -398,8 +413,13 @@ static int graph_for(qcbf_batch* b, double eps, hipGraphExec_t* exec) {
     }
     hipGraph_t graph = nullptr;
     BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_measure(b, b->dres, b->d_slot, true);
-    if (!rc) rc = enqueue_step(b, eps, cur);
+    int rc = QCBF_OK;
+    if (fusable(b)) {
+        rc = enqueue_step(b, eps, cur, b->dres, b->d_slot, true);
+    } else {
+        rc = enqueue_measure(b, b->dres, b->d_slot, true);
+        if (!rc) rc = enqueue_step(b, eps, cur);
+    }
     hipError_t e = hipStreamEndCapture(b->stream, &graph);
     if (rc) {
         if (graph) (void)hipGraphDestroy(graph);
@@ -447,11 +467,13 @@ extern "C" int qcbf_batch_flow_measure(qcbf_batch* b, double eps, int n_steps_gf
             t += eps;
             continue;
         }
-        int rc = enqueue_measure(b, b->dres, b->d_slot, true);
+        const bool fuse = fusable(b) && !last && n_rk_steps > 0;
+        int rc = fuse ? QCBF_OK : enqueue_measure(b, b->dres, b->d_slot, true);
         if (rc) return rc;
         if (!last)
             for (int k = 0; k < n_rk_steps; k++) {
-                rc = enqueue_step(b, eps, b->cur);
+                rc = fuse && k == 0 ? enqueue_step(b, eps, b->cur, b->dres, b->d_slot, true)
+                                    : enqueue_step(b, eps, b->cur);
                 if (rc) return rc;
                 b->cur ^= 1;
                 t += eps;
@@ -515,6 +537,10 @@ extern "C" int qcbf_batch_tune(qcbf_batch* b, int key, long value) {
             if (value < 1 || value > 16) return bfail(b, QCBF_ERR_ARG, "blocks per CU %ld not in 1..16", value);
             b->blocks_per_cu = (int)value;
             break;
+        case QCBF_TUNE_FUSED_MEASURE:
+            if (value != 0 && value != 1) return bfail(b, QCBF_ERR_ARG, "fused measure %ld not 0 or 1", value);
+            b->fuse_measure = value == 1;
+            break;
         case QCBF_TUNE_TILE_SHAPE: {
             if (value != -1 && (value < 0 || value >= 4096))
                 return bfail(b, QCBF_ERR_ARG, "tile shape %ld out of range", value);
@@ -539,6 +565,7 @@ extern "C" int qcbf_batch_get_tune(const qcbf_batch* b, int key, long* value) {
         case QCBF_TUNE_MAP_WIN: *value = b->bl.map_win; return QCBF_OK;
         case QCBF_TUNE_BLOCKS_PER_CU: *value = b->blocks_per_cu; return QCBF_OK;
         case QCBF_TUNE_TILE_SHAPE: *value = b->tile_shape; return QCBF_OK;
+        case QCBF_TUNE_FUSED_MEASURE: *value = b->fuse_measure ? 1 : 0; return QCBF_OK;
         default: return QCBF_ERR_ARG;
     }
 }

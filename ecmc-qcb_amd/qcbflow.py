@@ -30,8 +30,9 @@ QCBF_ERR_STATE = 6
 FP64 = 0
 FP32_MIXED = 1
 
-K_STAGE1, K_STAGE2, K_STAGE3, K_MEASURE = 0, 1, 2, 3
+K_STAGE1, K_STAGE2, K_STAGE3, K_MEASURE, K_STAGE1_MEASURE = 0, 1, 2, 3, 4
 TUNE_MAP_MODE, TUNE_MAP_WIN, TUNE_BLOCKS_PER_CU, TUNE_TILE_SHAPE = 0, 1, 2, 3
+TUNE_FUSED_MEASURE = 4
 
 
 def tile_code(bx, by, bz, bt):
@@ -39,7 +40,7 @@ def tile_code(bx, by, bz, bt):
     lg = [int(v).bit_length() - 1 for v in (bx, by, bz, bt)]
     assert all(1 << l == v for l, v in zip(lg, (bx, by, bz, bt))) and sum(lg) == 6
     return lg[0] + 8 * lg[1] + 64 * lg[2] + 512 * lg[3]
-K_COUNT = 4
+K_COUNT = 5
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)

@@ -61,7 +61,9 @@ enum qcbf_kernel_class {
     QCBF_K_STAGE2 = 1,  /* fused RK3 stage 2 */
     QCBF_K_STAGE3 = 2,  /* fused RK3 stage 3 */
     QCBF_K_MEASURE = 3, /* clover/plaquette/Q/E measurement + its reduction */
-    QCBF_K_COUNT = 4
+    QCBF_K_STAGE1_MEASURE = 4, /* stage 1 fused with the measurement of its input
+                                  (qcbf_flow_measure), reduction included */
+    QCBF_K_COUNT = 5
 };
 
 typedef struct qcbf_flow qcbf_flow;
@@ -181,6 +183,10 @@ enum qcbf_tune_key {
     QCBF_TUNE_MAP_WIN = 1,  /* mode 2: window (rounded down to a multiple of 8, at
                                least 8); mode 3: slab width in tiles; >= 1 */
     QCBF_TUNE_BLOCKS_PER_CU = 2, /* ensembles only: persistent-grid workgroups per CU */
+    QCBF_TUNE_FUSED_MEASURE = 4, /* qcbf_flow_measure fuses each measurement into the
+                                    next RK3 stage 1: 1 (default) or 0 (separate
+                                    kernels); results are bit-identical either way.
+                                    Needs the default tile shape. */
     QCBF_TUNE_TILE_SHAPE = 3     /* sites of one flow workgroup: -1 = 64 consecutive
                                     sites, else a 4-D box of extents 2^a x 2^b x 2^c x 2^d
                                     (a+b+c+d = 6) encoded a + 8b + 64c + 512d; the box

@@ -142,6 +142,25 @@ def test_batch_copy_from_single_lattice_handles(prec):
                 dev.copy_from(1, gf)
 
 
+@pytest.mark.parametrize("prec", [qcbflow.FP64, qcbflow.FP32_MIXED])
+def test_batch_fused_measurement_is_bit_identical(prec):
+    """Measurements riding on the next stage 1 (graphs on and off) give the
+    series of separate measurement kernels."""
+    dims = [(4, 4, 4, 4), (3, 5, 4, 7), (8, 8, 8, 8)]
+    fs = fields_for(dims, sweeps=1)
+    out = []
+    for fused, graph in [(1, True), (0, True), (1, False), (0, False)]:
+        with qcbflow.Ensemble(dims, 0.02, prec) as ens:
+            ens.tune(qcbflow.TUNE_FUSED_MEASURE, fused)
+            ens.use_graph(graph)
+            for c, f in enumerate(fs):
+                ens.copy(c, f)
+            out.append(ens.flow_measure(7, 1) + tuple(ens.field(c) for c in range(len(dims))))
+    for r in out[1:]:
+        for x, y in zip(out[0], r):
+            assert np.array_equal(x, y)
+
+
 def test_batch_errors():
     with pytest.raises(qcbflow.QcbfError) as ei:
         qcbflow.Ensemble([(4, 4, 4, 4), (4, 1, 4, 4)])

@@ -250,3 +250,32 @@ def test_flow_output_leaves_device_whole():
         ens.rk3_step(3)
         e0 = ens.field(0)
         assert np.array_equal(e0, out)
+
+
+@pytest.mark.parametrize("prec", [qcbflow.FP64, qcbflow.FP32_MIXED])
+@pytest.mark.parametrize("L", [(8, 8, 8, 8), (3, 5, 4, 7)])
+def test_fused_measurement_is_bit_identical(prec, L):
+    """qcbf_flow_measure measures inside the next RK3 stage 1; the series is
+    bit-for-bit the one of separate measurement kernels and of measuring step
+    by step (a lattice whose volume is not a multiple of 64 included)."""
+    say(f"fused measurement {L} precision {prec}")
+    f = thermal(L, 5, 2)
+    with qcbflow.GradientFlow(L, 0.02, prec) as a, qcbflow.GradientFlow(L, 0.02, prec) as b, \
+            qcbflow.GradientFlow(L, 0.02, prec) as c:
+        for h in (a, b, c):
+            h.copy(f)
+        assert a.get_tune(qcbflow.TUNE_FUSED_MEASURE) == 1
+        b.tune(qcbflow.TUNE_FUSED_MEASURE, 0)
+        ta, pa = a.flow_measure(6, 1)
+        tb, pb = b.flow_measure(6, 1)
+        assert np.array_equal(ta, tb) and np.array_equal(pa, pb)
+        assert np.array_equal(a.field(), b.field())
+        for k in range(6):
+            p, e, q = c.measure()
+            assert (p, e, q) == (pa[k], ta[k, 2], ta[k, 1]), k
+            if k < 5:
+                c.rk3_step(1)
+        assert np.array_equal(a.field(), c.field())
+        ta2, pa2 = a.flow_measure(4, 2)  # measurement every second step
+        tb2, pb2 = b.flow_measure(4, 2)
+        assert np.array_equal(ta2, tb2) and np.array_equal(pa2, pb2)
