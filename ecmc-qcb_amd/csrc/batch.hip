@@ -1,11 +1,12 @@
-// batch.hip -- persistent ensemble kernels: one launch per RK3 stage (and one
+// batch.hip -- ensemble kernels: one launch per RK3 stage (and one
 // measurement + one reduction per measurement) for a whole ensemble of
 // lattices of uneven volumes.
 //
 // The persistent grid is sized to the resident capacity of the GPU; workgroup b
 // visits virtual slots v = b, b + grid, b + 2 grid, ... and each slot maps to a
 // tile of the flat (configuration, tile) list through the XCD-aware order of
-// tile_of_block.  The per-link and per-site work is exactly that of the
+// tile_of_block.  With a grid of zero the same kernels run one workgroup per
+// slot instead (for_each_slot).  The per-link and per-site work is exactly that of the
 // single-lattice kernels (flow_body.hpp, measure_body.hpp).
 #include "batch.hpp"
 #include "flow_body.hpp"
@@ -15,35 +16,87 @@
 
 namespace qcb {
 
+// Slots of this workgroup: b, b + grid, b + 2 grid, ... on the persistent
+// grid, or the one slot b when the grid has a workgroup per slot.  The flat
+// form exists because the loop of the persistent form costs the register
+// allocator up to 40 VGPRs in these kernels (a wavefront per SIMD); which of
+// the two is faster for an ensemble is the autotuner's call.  The per-slot
+// bodies are functions with __restrict__ pointer parameters, so the loads of
+// the input field may move past the stores of the output as in the
+// single-lattice kernels.
+#define QCB_FOR_EACH_SLOT(PERSIST, nslots, call)                             \
+    do {                                                                     \
+        if constexpr (PERSIST) {                                             \
+            for (unsigned v = blockIdx.x; v < (nslots); v += gridDim.x) call; \
+        } else if (blockIdx.x < (nslots)) {                                  \
+            const unsigned v = blockIdx.x;                                   \
+            call;                                                            \
+        }                                                                    \
+    } while (0)
+
 template <prec P, int STAGE>
-__global__ __launch_bounds__(256) void k_batch_stage(const typename storage<P>::vec* __restrict__ Uin,
+__device__ __forceinline__ void batch_stage_slot(unsigned v,
+                                                 const typename storage<P>::vec* __restrict__ Uin,
+                                                 typename storage<P>::vec* __restrict__ Uout,
+                                                 typename storage<P>::avec* __restrict__ acc,
+                                                 const cfg_desc* __restrict__ cfgs,
+                                                 const unsigned* __restrict__ tile_cfg,
+                                                 unsigned ntiles, int map_mode, unsigned map_win,
+                                                 double cz, double ca, bool& w) {
+    const unsigned tile = tile_of_block(v, ntiles, map_mode, map_win);
+    if (tile >= ntiles) return;
+    const cfg_desc& c = cfgs[tile_cfg[tile]];
+    unsigned co[4];
+    const unsigned s = tile_site(c.g, tile - c.tile0, threadIdx.x, co);
+    if (s >= c.g.V) return;
+    const nbr n = neighbours_at(c.g, co);
+    const auto f = storage<P>::view(Uin + c.u_off, c.g.V);
+    auto* out = Uout + c.u_off;
+    auto* a = acc + c.a_off;
+    switch (threadIdx.y) {
+        case 0: stage_body<P, STAGE, 0>(f, out, a, s, n, cz, ca, w); break;
+        case 1: stage_body<P, STAGE, 1>(f, out, a, s, n, cz, ca, w); break;
+        case 2: stage_body<P, STAGE, 2>(f, out, a, s, n, cz, ca, w); break;
+        default: stage_body<P, STAGE, 3>(f, out, a, s, n, cz, ca, w); break;
+    }
+}
+
+// Wavefronts per SIMD of the one-workgroup-per-tile stage kernel: the
+// single-lattice kernel's count, which the register allocator reaches here
+// only when asked (FP32: 76 VGPRs instead of 113).
+template <prec P>
+constexpr int kFlatStageWaves = P == prec::f64 ? 3 : 6;
+
+template <prec P, int STAGE, bool PERSIST>
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(PERSIST ? 1 : kFlatStageWaves<P>))) void k_batch_stage(const typename storage<P>::vec* __restrict__ Uin,
                                                      typename storage<P>::vec* __restrict__ Uout,
                                                      typename storage<P>::avec* __restrict__ acc,
                                                      const cfg_desc* __restrict__ cfgs,
                                                      const unsigned* __restrict__ tile_cfg,
                                                      unsigned ntiles, int map_mode, unsigned map_win,
                                                      double cz, double ca, int* __restrict__ warn) {
-    const unsigned nslots = grid_for_tiles(ntiles, map_mode, map_win);
     bool w = false;
-    for (unsigned v = blockIdx.x; v < nslots; v += gridDim.x) {
-        const unsigned tile = tile_of_block(v, ntiles, map_mode, map_win);
-        if (tile >= ntiles) continue;
-        const cfg_desc& c = cfgs[tile_cfg[tile]];
-        unsigned co[4];
-        const unsigned s = tile_site(c.g, tile - c.tile0, threadIdx.x, co);
-        if (s >= c.g.V) continue;
-        const nbr n = neighbours_at(c.g, co);
-        const auto f = storage<P>::view(Uin + c.u_off, c.g.V);
-        auto* out = Uout + c.u_off;
-        auto* a = acc + c.a_off;
-        switch (threadIdx.y) {
-            case 0: stage_body<P, STAGE, 0>(f, out, a, s, n, cz, ca, w); break;
-            case 1: stage_body<P, STAGE, 1>(f, out, a, s, n, cz, ca, w); break;
-            case 2: stage_body<P, STAGE, 2>(f, out, a, s, n, cz, ca, w); break;
-            default: stage_body<P, STAGE, 3>(f, out, a, s, n, cz, ca, w); break;
-        }
-    }
+    const unsigned nslots = grid_for_tiles(ntiles, map_mode, map_win);
+    QCB_FOR_EACH_SLOT(PERSIST, nslots,
+                      (batch_stage_slot<P, STAGE>(v, Uin, Uout, acc, cfgs, tile_cfg, ntiles,
+                                                  map_mode, map_win, cz, ca, w)));
     if (w) atomicOr(warn, 1);
+}
+
+template <prec P, int STAGE>
+static void batch_stage_kernel(const batch_launch& bl, unsigned nslots, hipStream_t st,
+                               const typename storage<P>::vec* in, typename storage<P>::vec* out,
+                               typename storage<P>::avec* a, const cfg_desc* cfgs,
+                               const unsigned* tile_cfg, unsigned ntiles, double cz, double ca,
+                               int* warn) {
+    const dim3 block(kSitesPerBlock, 4);
+    if (bl.grid)
+        k_batch_stage<P, STAGE, true><<<bl.grid, block, 0, st>>>(
+            in, out, a, cfgs, tile_cfg, ntiles, bl.map_mode, bl.map_win, cz, ca, warn);
+    else
+        k_batch_stage<P, STAGE, false><<<nslots, block, 0, st>>>(
+            in, out, a, cfgs, tile_cfg, ntiles, bl.map_mode, bl.map_win, cz, ca, warn);
 }
 
 template <prec P>
@@ -55,22 +108,11 @@ static hipError_t batch_stage_launch(int stage, const void* Uin, void* Uout, voi
     const auto* in = static_cast<const typename S::vec*>(Uin);
     auto* out = static_cast<typename S::vec*>(Uout);
     auto* a = static_cast<typename S::avec*>(acc);
-    const dim3 block(kSitesPerBlock, 4);
-    const int mm = bl.map_mode;
-    const unsigned mw = bl.map_win;
+    const unsigned nslots = grid_for_tiles(ntiles, bl.map_mode, bl.map_win);
     switch (stage) {
-        case 1:
-            k_batch_stage<P, 1><<<bl.grid, block, 0, st>>>(in, out, a, cfgs, tile_cfg, ntiles, mm,
-                                                           mw, cz, ca, warn);
-            break;
-        case 2:
-            k_batch_stage<P, 2><<<bl.grid, block, 0, st>>>(in, out, a, cfgs, tile_cfg, ntiles, mm,
-                                                           mw, cz, ca, warn);
-            break;
-        case 3:
-            k_batch_stage<P, 3><<<bl.grid, block, 0, st>>>(in, out, a, cfgs, tile_cfg, ntiles, mm,
-                                                           mw, cz, ca, warn);
-            break;
+        case 1: batch_stage_kernel<P, 1>(bl, nslots, st, in, out, a, cfgs, tile_cfg, ntiles, cz, ca, warn); break;
+        case 2: batch_stage_kernel<P, 2>(bl, nslots, st, in, out, a, cfgs, tile_cfg, ntiles, cz, ca, warn); break;
+        case 3: batch_stage_kernel<P, 3>(bl, nslots, st, in, out, a, cfgs, tile_cfg, ntiles, cz, ca, warn); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -92,44 +134,55 @@ hipError_t launch_batch_stage(prec p, int stage, const void* Uin, void* Uout, vo
 // Same per-site code and workgroup reduction as k_measure_clover (measure.hip):
 // wavefront mu forms its clover leaves, LDS joins the two halves of each plane.
 template <prec P>
-__global__ __launch_bounds__(256) void k_batch_measure(const typename storage<P>::vec* __restrict__ U,
-                                                       const cfg_desc* __restrict__ cfgs,
-                                                       const unsigned* __restrict__ tile_cfg,
-                                                       unsigned ntiles, int map_mode,
-                                                       unsigned map_win,
-                                                       double* __restrict__ partials) {
+__device__ __forceinline__ void batch_measure_slot(unsigned v,
+                                                   const typename storage<P>::vec* __restrict__ U,
+                                                   const cfg_desc* __restrict__ cfgs,
+                                                   const unsigned* __restrict__ tile_cfg,
+                                                   unsigned ntiles, int map_mode, unsigned map_win,
+                                                   double* __restrict__ partials,
+                                                   typename storage<P>::real* lds, double* red) {
     using FT = typename storage<P>::field;
-    __shared__ typename FT::real lds[kCloverSlots * 9 * 64];
+    const unsigned tile = tile_of_block(v, ntiles, map_mode, map_win);
+    if (tile >= ntiles) return;  // uniform per workgroup: no barrier is skipped by part of it
+    const unsigned lane = threadIdx.x;
+    const cfg_desc& c = cfgs[tile_cfg[tile]];
+    const unsigned s = (tile - c.tile0) * kSitesPerBlock + lane;
+    const bool valid = s < c.g.V;
+    const FT f = storage<P>::view(U + c.u_off, c.g.V);
+    const nbr n = neighbours(c.g, valid ? s : 0u);
+    double r[3] = {0.0, 0.0, 0.0};
+    if (valid) {
+        switch (threadIdx.y) {
+            case 0: r[0] = clover_wave<FT, 0>(f, s, n, lds, lane); break;
+            case 1: r[0] = clover_wave<FT, 1>(f, s, n, lds, lane); break;
+            case 2: r[0] = clover_wave<FT, 2>(f, s, n, lds, lane); break;
+            default: r[0] = clover_wave<FT, 3>(f, s, n, lds, lane); break;
+        }
+    }
+    __syncthreads();
+    if (valid) clover_pair_terms(lds, lane, (int)threadIdx.y, r[1], r[2]);
+    block_sum<3>(r, red, 4);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        partials[3 * tile + 0] = r[0];
+        partials[3 * tile + 1] = r[1];
+        partials[3 * tile + 2] = r[2];
+    }
+    __syncthreads();  // lds and red are reused by the next tile
+}
+
+template <prec P, bool PERSIST>
+__global__ __launch_bounds__(256, 2) void k_batch_measure(const typename storage<P>::vec* __restrict__ U,
+                                                          const cfg_desc* __restrict__ cfgs,
+                                                          const unsigned* __restrict__ tile_cfg,
+                                                          unsigned ntiles, int map_mode,
+                                                          unsigned map_win,
+                                                          double* __restrict__ partials) {
+    __shared__ typename storage<P>::real lds[kCloverSlots * 9 * 64];
     __shared__ double red[4 * 3];
     const unsigned nslots = grid_for_tiles(ntiles, map_mode, map_win);
-    const unsigned lane = threadIdx.x;
-    for (unsigned v0 = blockIdx.x; v0 < nslots; v0 += gridDim.x) {  // uniform per workgroup
-        const unsigned tile = tile_of_block(v0, ntiles, map_mode, map_win);
-        if (tile >= ntiles) continue;
-        const cfg_desc& c = cfgs[tile_cfg[tile]];
-        const unsigned s = (tile - c.tile0) * kSitesPerBlock + lane;
-        const bool valid = s < c.g.V;
-        const FT f = storage<P>::view(U + c.u_off, c.g.V);
-        const nbr n = neighbours(c.g, valid ? s : 0u);
-        double r[3] = {0.0, 0.0, 0.0};
-        if (valid) {
-            switch (threadIdx.y) {
-                case 0: r[0] = clover_wave<FT, 0>(f, s, n, lds, lane); break;
-                case 1: r[0] = clover_wave<FT, 1>(f, s, n, lds, lane); break;
-                case 2: r[0] = clover_wave<FT, 2>(f, s, n, lds, lane); break;
-                default: r[0] = clover_wave<FT, 3>(f, s, n, lds, lane); break;
-            }
-        }
-        __syncthreads();
-        if (valid) clover_pair_terms(lds, lane, (int)threadIdx.y, r[1], r[2]);
-        block_sum<3>(r, red, 4);
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            partials[3 * tile + 0] = r[0];
-            partials[3 * tile + 1] = r[1];
-            partials[3 * tile + 2] = r[2];
-        }
-        __syncthreads();  // lds and red are reused by the next tile
-    }
+    QCB_FOR_EACH_SLOT(PERSIST, nslots,
+                      (batch_measure_slot<P>(v, U, cfgs, tile_cfg, ntiles, map_mode, map_win,
+                                             partials, lds, red)));
 }
 
 // RK3 stage 1 fused with the measurement of its input, for every member
@@ -147,46 +200,57 @@ __device__ inline double batch_stage1_measure_wave(const typename storage<P>::fi
 }
 
 template <prec P>
-__global__ __launch_bounds__(256) void k_batch_stage1_measure(
+__device__ __forceinline__ void batch_stage1_measure_slot(
+    unsigned v, const typename storage<P>::vec* __restrict__ Uin,
+    typename storage<P>::vec* __restrict__ Uout, typename storage<P>::avec* __restrict__ acc,
+    const cfg_desc* __restrict__ cfgs, const unsigned* __restrict__ tile_cfg, unsigned ntiles,
+    int map_mode, unsigned map_win, double cz, bool& w, double* __restrict__ partials,
+    typename storage<P>::real* lds, double* red) {
+    using FT = typename storage<P>::field;
+    const unsigned tile = tile_of_block(v, ntiles, map_mode, map_win);
+    if (tile >= ntiles) return;  // uniform per workgroup: no barrier is skipped by part of it
+    const unsigned lane = threadIdx.x;
+    const cfg_desc& c = cfgs[tile_cfg[tile]];
+    const unsigned s = (tile - c.tile0) * kSitesPerBlock + lane;
+    const bool valid = s < c.g.V;
+    const FT f = storage<P>::view(Uin + c.u_off, c.g.V);
+    const nbr n = neighbours(c.g, valid ? s : 0u);
+    auto* out = Uout + c.u_off;
+    auto* a = acc + c.a_off;
+    double r[3] = {0.0, 0.0, 0.0};
+    if (valid) {
+        switch (threadIdx.y) {
+            case 0: r[0] = batch_stage1_measure_wave<P, 0>(f, out, a, s, n, cz, w, lds, lane); break;
+            case 1: r[0] = batch_stage1_measure_wave<P, 1>(f, out, a, s, n, cz, w, lds, lane); break;
+            case 2: r[0] = batch_stage1_measure_wave<P, 2>(f, out, a, s, n, cz, w, lds, lane); break;
+            default: r[0] = batch_stage1_measure_wave<P, 3>(f, out, a, s, n, cz, w, lds, lane); break;
+        }
+    }
+    __syncthreads();
+    if (valid) clover_pair_terms(lds, lane, (int)threadIdx.y, r[1], r[2]);
+    block_sum<3>(r, red, 4);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        partials[3 * tile + 0] = r[0];
+        partials[3 * tile + 1] = r[1];
+        partials[3 * tile + 2] = r[2];
+    }
+    __syncthreads();  // lds and red are reused by the next tile
+}
+
+template <prec P, bool PERSIST>
+__global__ __launch_bounds__(256, 2) void k_batch_stage1_measure(
     const typename storage<P>::vec* __restrict__ Uin, typename storage<P>::vec* __restrict__ Uout,
     typename storage<P>::avec* __restrict__ acc, const cfg_desc* __restrict__ cfgs,
     const unsigned* __restrict__ tile_cfg, unsigned ntiles, int map_mode, unsigned map_win,
     double cz, int* __restrict__ warn, double* __restrict__ partials) {
-    using FT = typename storage<P>::field;
-    __shared__ typename FT::real lds[kCloverSlots * 9 * 64];
+    __shared__ typename storage<P>::real lds[kCloverSlots * 9 * 64];
     __shared__ double red[4 * 3];
-    const unsigned nslots = grid_for_tiles(ntiles, map_mode, map_win);
-    const unsigned lane = threadIdx.x;
     bool w = false;
-    for (unsigned v0 = blockIdx.x; v0 < nslots; v0 += gridDim.x) {  // uniform per workgroup
-        const unsigned tile = tile_of_block(v0, ntiles, map_mode, map_win);
-        if (tile >= ntiles) continue;
-        const cfg_desc& c = cfgs[tile_cfg[tile]];
-        const unsigned s = (tile - c.tile0) * kSitesPerBlock + lane;
-        const bool valid = s < c.g.V;
-        const FT f = storage<P>::view(Uin + c.u_off, c.g.V);
-        const nbr n = neighbours(c.g, valid ? s : 0u);
-        auto* out = Uout + c.u_off;
-        auto* a = acc + c.a_off;
-        double r[3] = {0.0, 0.0, 0.0};
-        if (valid) {
-            switch (threadIdx.y) {
-                case 0: r[0] = batch_stage1_measure_wave<P, 0>(f, out, a, s, n, cz, w, lds, lane); break;
-                case 1: r[0] = batch_stage1_measure_wave<P, 1>(f, out, a, s, n, cz, w, lds, lane); break;
-                case 2: r[0] = batch_stage1_measure_wave<P, 2>(f, out, a, s, n, cz, w, lds, lane); break;
-                default: r[0] = batch_stage1_measure_wave<P, 3>(f, out, a, s, n, cz, w, lds, lane); break;
-            }
-        }
-        __syncthreads();
-        if (valid) clover_pair_terms(lds, lane, (int)threadIdx.y, r[1], r[2]);
-        block_sum<3>(r, red, 4);
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            partials[3 * tile + 0] = r[0];
-            partials[3 * tile + 1] = r[1];
-            partials[3 * tile + 2] = r[2];
-        }
-        __syncthreads();  // lds and red are reused by the next tile
-    }
+    const unsigned nslots = grid_for_tiles(ntiles, map_mode, map_win);
+    QCB_FOR_EACH_SLOT(PERSIST, nslots,
+                      (batch_stage1_measure_slot<P>(v, Uin, Uout, acc, cfgs, tile_cfg, ntiles,
+                                                    map_mode, map_win, cz, w, partials, lds,
+                                                    red)));
     if (w) atomicOr(warn, 1);
 }
 
@@ -228,14 +292,21 @@ hipError_t launch_batch_measure(prec p, const void* U, const cfg_desc* cfgs,
                                 unsigned* slot_ctr, bool bump, hipStream_t st) {
     (void)hipGetLastError();  // drop a stale error of an earlier call
     const dim3 block(kSitesPerBlock, 4);
-    if (p == prec::f64)
-        k_batch_measure<prec::f64><<<bl.grid, block, 0, st>>>(
-            static_cast<const double2*>(U), cfgs, tile_cfg, ntiles, bl.map_mode, bl.map_win,
-            partials);
+    const unsigned nslots = grid_for_tiles(ntiles, bl.map_mode, bl.map_win);
+    const auto* u64 = static_cast<const double2*>(U);
+    const auto* u32 = static_cast<const float4*>(U);
+    if (p == prec::f64 && bl.grid)
+        k_batch_measure<prec::f64, true><<<bl.grid, block, 0, st>>>(
+            u64, cfgs, tile_cfg, ntiles, bl.map_mode, bl.map_win, partials);
+    else if (p == prec::f64)
+        k_batch_measure<prec::f64, false><<<nslots, block, 0, st>>>(
+            u64, cfgs, tile_cfg, ntiles, bl.map_mode, bl.map_win, partials);
+    else if (bl.grid)
+        k_batch_measure<prec::f32m, true><<<bl.grid, block, 0, st>>>(
+            u32, cfgs, tile_cfg, ntiles, bl.map_mode, bl.map_win, partials);
     else
-        k_batch_measure<prec::f32m><<<bl.grid, block, 0, st>>>(
-            static_cast<const float4*>(U), cfgs, tile_cfg, ntiles, bl.map_mode, bl.map_win,
-            partials);
+        k_batch_measure<prec::f32m, false><<<nslots, block, 0, st>>>(
+            u32, cfgs, tile_cfg, ntiles, bl.map_mode, bl.map_win, partials);
     k_batch_reduce<<<ncfg, 256, 0, st>>>(partials, cfgs, ncfg, out, slot_ctr);
     if (bump && slot_ctr) k_bump<<<1, 1, 0, st>>>(slot_ctr);
     return hipGetLastError();
@@ -248,15 +319,29 @@ hipError_t launch_batch_stage1_measure(prec p, const void* Uin, void* Uout, void
                                        unsigned* slot_ctr, bool bump, hipStream_t st) {
     (void)hipGetLastError();  // drop a stale error of an earlier call
     const dim3 block(kSitesPerBlock, 4);
-    if (p == prec::f64)
-        k_batch_stage1_measure<prec::f64><<<bl.grid, block, 0, st>>>(
-            static_cast<const double2*>(Uin), static_cast<double2*>(Uout),
-            static_cast<double2*>(acc), cfgs, tile_cfg, ntiles, bl.map_mode, bl.map_win, cz, warn,
-            partials);
-    else
-        k_batch_stage1_measure<prec::f32m><<<bl.grid, block, 0, st>>>(
-            static_cast<const float4*>(Uin), static_cast<float4*>(Uout), static_cast<float4*>(acc),
-            cfgs, tile_cfg, ntiles, bl.map_mode, bl.map_win, cz, warn, partials);
+    const unsigned nslots = grid_for_tiles(ntiles, bl.map_mode, bl.map_win);
+    const unsigned grid = bl.grid ? bl.grid : nslots;
+    if (p == prec::f64) {
+        const auto* in = static_cast<const double2*>(Uin);
+        auto* out = static_cast<double2*>(Uout);
+        auto* a = static_cast<double2*>(acc);
+        if (bl.grid)
+            k_batch_stage1_measure<prec::f64, true><<<grid, block, 0, st>>>(
+                in, out, a, cfgs, tile_cfg, ntiles, bl.map_mode, bl.map_win, cz, warn, partials);
+        else
+            k_batch_stage1_measure<prec::f64, false><<<grid, block, 0, st>>>(
+                in, out, a, cfgs, tile_cfg, ntiles, bl.map_mode, bl.map_win, cz, warn, partials);
+    } else {
+        const auto* in = static_cast<const float4*>(Uin);
+        auto* out = static_cast<float4*>(Uout);
+        auto* a = static_cast<float4*>(acc);
+        if (bl.grid)
+            k_batch_stage1_measure<prec::f32m, true><<<grid, block, 0, st>>>(
+                in, out, a, cfgs, tile_cfg, ntiles, bl.map_mode, bl.map_win, cz, warn, partials);
+        else
+            k_batch_stage1_measure<prec::f32m, false><<<grid, block, 0, st>>>(
+                in, out, a, cfgs, tile_cfg, ntiles, bl.map_mode, bl.map_win, cz, warn, partials);
+    }
     k_batch_reduce<<<ncfg, 256, 0, st>>>(partials, cfgs, ncfg, out, slot_ctr);
     if (bump && slot_ctr) k_bump<<<1, 1, 0, st>>>(slot_ctr);
     return hipGetLastError();

@@ -23,7 +23,7 @@ struct cfg_desc {
 
 // Launch shape of the persistent kernels.
 struct batch_launch {
-    unsigned grid;      // workgroups of the persistent grid
+    unsigned grid;      // workgroups of the persistent grid; 0: one workgroup per tile
     int map_mode;       // tile order (tile_of_block)
     unsigned map_win;
 };

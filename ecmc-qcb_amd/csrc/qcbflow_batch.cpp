@@ -534,7 +534,7 @@ extern "C" int qcbf_batch_tune(qcbf_batch* b, int key, long value) {
             b->bl.map_win = (unsigned)value;
             break;
         case QCBF_TUNE_BLOCKS_PER_CU:
-            if (value < 1 || value > 16) return bfail(b, QCBF_ERR_ARG, "blocks per CU %ld not in 1..16", value);
+            if (value < 0 || value > 16) return bfail(b, QCBF_ERR_ARG, "blocks per CU %ld not in 0..16", value);
             b->blocks_per_cu = (int)value;
             break;
         case QCBF_TUNE_FUSED_MEASURE:
@@ -619,7 +619,7 @@ extern "C" int qcbf_batch_autotune(qcbf_batch* b, double eps, int trial_steps, l
     std::vector<cand> cands;
     const std::pair<int, unsigned> maps[] = {{0, 64}, {2, 256}, {2, 1024}};
     for (auto m : maps)
-        for (int bpc : {1, 2, 4})
+        for (int bpc : {0, 1, 2, 3, 4})
             for (long sh : {-1L, 594L, 83L}) cands.push_back({m.first, m.second, bpc, sh});
     cand best = {b->bl.map_mode, b->bl.map_win, b->blocks_per_cu, b->tile_shape};
     auto apply_shape = [&](long code) -> hipError_t {

@@ -145,13 +145,16 @@ def test_batch_copy_from_single_lattice_handles(prec):
 @pytest.mark.parametrize("prec", [qcbflow.FP64, qcbflow.FP32_MIXED])
 def test_batch_fused_measurement_is_bit_identical(prec):
     """Measurements riding on the next stage 1 (graphs on and off) give the
-    series of separate measurement kernels."""
+    series of separate measurement kernels, on the persistent grid and with
+    one workgroup per tile."""
     dims = [(4, 4, 4, 4), (3, 5, 4, 7), (8, 8, 8, 8)]
     fs = fields_for(dims, sweeps=1)
     out = []
-    for fused, graph in [(1, True), (0, True), (1, False), (0, False)]:
+    for fused, graph, bpc in [(1, True, 2), (0, True, 2), (1, False, 2), (0, False, 2),
+                              (1, True, 0), (0, False, 0), (1, False, 3)]:
         with qcbflow.Ensemble(dims, 0.02, prec) as ens:
             ens.tune(qcbflow.TUNE_FUSED_MEASURE, fused)
+            ens.tune(qcbflow.TUNE_BLOCKS_PER_CU, bpc)
             ens.use_graph(graph)
             for c, f in enumerate(fs):
                 ens.copy(c, f)
@@ -169,4 +172,4 @@ def test_batch_errors():
         with pytest.raises(qcbflow.QcbfError):
             ens.hot_start(3, 1)
         with pytest.raises(qcbflow.QcbfError):
-            ens.tune(qcbflow.TUNE_BLOCKS_PER_CU, 0)
+            ens.tune(qcbflow.TUNE_BLOCKS_PER_CU, 17)
