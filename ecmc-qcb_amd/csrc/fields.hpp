@@ -34,12 +34,19 @@ enum class prec { f64 = 0, f32m = 1 };
 // ---- FP32 two-row compressed field -------------------------------------------
 struct field32c {
     using real = float;
+    using word = float4;
     static constexpr bool stream_rows = false;
+    static constexpr int kWords = 3;  // two rows as three float4
     const float4* __restrict__ p;
     unsigned V;
-    __device__ inline mat3<float> link(int mu, unsigned s) const {
+    __device__ inline void load_words(int mu, unsigned s, word (&w)[kWords]) const {
         const float4* b = p + (size_t)mu * 3 * V + s;
-        const float4 a = b[0], c = b[V], d = b[2 * (size_t)V];
+        w[0] = b[0];
+        w[1] = b[V];
+        w[2] = b[2 * (size_t)V];
+    }
+    __device__ static inline mat3<float> from_words(const word (&w)[kWords]) {
+        const float4 a = w[0], c = w[1], d = w[2];
         mat3<float> m;
         m.e[0][0] = {a.x, a.y};
         m.e[0][1] = {a.z, a.w};
@@ -51,6 +58,11 @@ struct field32c {
         m.e[2][1] = conj(m.e[0][2] * m.e[1][0] - m.e[0][0] * m.e[1][2]);
         m.e[2][2] = conj(m.e[0][0] * m.e[1][1] - m.e[0][1] * m.e[1][0]);
         return m;
+    }
+    __device__ inline mat3<float> link(int mu, unsigned s) const {
+        word w[kWords];
+        load_words(mu, s, w);
+        return from_words(w);
     }
 };
 

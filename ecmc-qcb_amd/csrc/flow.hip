@@ -29,12 +29,15 @@
 
 namespace qcb {
 
-#ifndef QCB_STAGE_WAVES_PER_EU
-#define QCB_STAGE_WAVES_PER_EU 1
-#endif
+// Wavefronts per SIMD the stage kernel is compiled for: FP64 3 (its natural
+// 148 VGPRs), FP32-mixed 6 (76).  (Staging the site's own links in LDS for
+// the four direction wavefronts was measured slower: the barrier costs more
+// than the L1 hits it replaces.)
+template <prec P>
+constexpr int kStageWaves = P == prec::f64 ? 3 : 6;
 
 template <prec P, int STAGE>
-__global__ __launch_bounds__(256, QCB_STAGE_WAVES_PER_EU) void k_rk3_stage(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kStageWaves<P>))) void k_rk3_stage(
     const typename storage<P>::vec* __restrict__ Uin, typename storage<P>::vec* __restrict__ Uout,
     typename storage<P>::avec* __restrict__ acc, lattice g, double cz, double ca,
     int* __restrict__ warn) {

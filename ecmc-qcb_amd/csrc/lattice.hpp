@@ -224,17 +224,21 @@ __host__ __device__ inline unsigned stage_tiles(const lattice& g) {
 #endif
 struct field64 {
     using real = double;
+    using word = double2;
     static constexpr bool stream_rows = QCB_F64_STREAM_ROWS && !QCB_F64_RECONSTRUCT;
+    // words read per link: two rows (third rebuilt) or all nine entries
+    static constexpr int kWords = QCB_F64_RECONSTRUCT ? 6 : 9;
     const double2* __restrict__ p;
     unsigned V;
-    __device__ inline mat3<double> link(int mu, unsigned s) const {
-        mat3<double> m;
+    __device__ inline void load_words(int mu, unsigned s, word (&w)[kWords]) const {
         const double2* b = p + (size_t)mu * 9 * V + s;
 #pragma unroll
-        for (int k = 0; k < (QCB_F64_RECONSTRUCT ? 6 : 9); k++) {
-            double2 v = b[(size_t)k * V];
-            m.e[k / 3][k % 3] = {v.x, v.y};
-        }
+        for (int k = 0; k < kWords; k++) w[k] = b[(size_t)k * V];
+    }
+    __device__ static inline mat3<double> from_words(const word (&w)[kWords]) {
+        mat3<double> m;
+#pragma unroll
+        for (int k = 0; k < kWords; k++) m.e[k / 3][k % 3] = {w[k].x, w[k].y};
 #if QCB_F64_RECONSTRUCT
         // third row of an SU(3) matrix: conj(row0 x row1)
         m.e[2][0] = conj(m.e[0][1] * m.e[1][2] - m.e[0][2] * m.e[1][1]);
@@ -242,6 +246,11 @@ struct field64 {
         m.e[2][2] = conj(m.e[0][0] * m.e[1][1] - m.e[0][1] * m.e[1][0]);
 #endif
         return m;
+    }
+    __device__ inline mat3<double> link(int mu, unsigned s) const {
+        word w[kWords];
+        load_words(mu, s, w);
+        return from_words(w);
     }
     // row i of U_mu(s)
     __device__ inline void row(int mu, unsigned s, int i, cpx<double> (&r)[3]) const {
