@@ -79,7 +79,7 @@ hipError_t launch_complete_rows(prec p, void* U, unsigned V, hipStream_t st);
 
 // n_sweeps checkerboard Cabibbo-Marinari heatbath sweeps at beta on an FP64
 // field with even extents; sweep numbers sweep0.. key the random streams.
-// *stuck is set if a Kennedy-Pendleton draw hit its retry bound.
+// *stuck is set if an SU(2) draw hit its retry bound.
 hipError_t launch_heatbath_f64(double2* U, const lattice& g, double beta, int n_sweeps,
                                uint64_t seed, unsigned sweep0, int* stuck, hipStream_t st);
 
