@@ -30,9 +30,10 @@
 namespace qcb {
 
 // Wavefronts per SIMD the stage kernel is compiled for: FP64 3 (its natural
-// 148 VGPRs), FP32-mixed 6 (76).  (Staging the site's own links in LDS for
-// the four direction wavefronts was measured slower: the barrier costs more
-// than the L1 hits it replaces.)
+// 150 VGPRs), FP32-mixed 6 (76).  Two ways to more FP64 wavefronts were
+// measured slower on MI355X: staging the site's own links in LDS for the four
+// direction wavefronts (the barrier costs more than the L1 hits it replaces),
+// and keeping the staple sum in LDS (130 VGPRs at 3 waves, 128 + spills at 4).
 template <prec P>
 constexpr int kStageWaves = P == prec::f64 ? 3 : 6;
 
