@@ -73,6 +73,9 @@ hipError_t launch_alg_to_aos(prec p, const void* alg, double* aos, const lattice
                              hipStream_t st);
 hipError_t launch_hot_start(prec p, void* U, const lattice& g, uint64_t seed, hipStream_t st);
 hipError_t launch_cold_start(prec p, void* U, const lattice& g, hipStream_t st);
+// out_mu(x) = in_mu(x - shift) on the periodic lattice, 0 <= shift[d] < L[d].
+hipError_t launch_shift(prec p, const void* in, void* out, const lattice& g, const int shift[4],
+                        hipStream_t st);
 // Rebuild the third row of every link of an FP64 flow output in place (no-op
 // for the FP32-mixed form, which never stores it).
 hipError_t launch_complete_rows(prec p, void* U, unsigned V, hipStream_t st);

@@ -243,6 +243,46 @@ __global__ __launch_bounds__(256) void k_cold_start(typename storage<P>::vec* __
         store_link32c(U, V, threadIdx.y, s, m);
 }
 
+// Global translation U'_mu(x) = U_mu(x - shift) on the periodic lattice, every
+// direction and every stored word of the links (mpi::shift::shift_field,
+// src/mpi/Shift.cpp:109-160: the reference moves slices between ranks and
+// shifts the rest locally; on one device it is a single gather).
+template <prec P>
+__global__ __launch_bounds__(256) void k_shift(const typename storage<P>::vec* __restrict__ in,
+                                               typename storage<P>::vec* __restrict__ out,
+                                               lattice g, int4 sh) {
+    const unsigned s = blockIdx.x * kSitesPerBlock + threadIdx.x;
+    if (s >= g.V) return;
+    unsigned r = s, src = 0;
+    const int shv[4] = {sh.x, sh.y, sh.z, sh.w};
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+        const unsigned L = (unsigned)g.L[d];
+        const unsigned c = d < 3 ? r % L : r;
+        r = d < 3 ? r / L : 0u;
+        src += ((c + L - (unsigned)shv[d]) % L) * g.stride[d];
+    }
+    constexpr int W = storage<P>::link_words;
+    const int mu = threadIdx.y;
+#pragma unroll
+    for (int k = 0; k < W; k++)
+        out[(size_t)(mu * W + k) * g.V + s] = in[(size_t)(mu * W + k) * g.V + src];
+}
+
+hipError_t launch_shift(prec p, const void* in, void* out, const lattice& g, const int shift[4],
+                        hipStream_t st) {
+    (void)hipGetLastError();  // drop a stale error of an earlier call
+    const int4 sh = make_int4(shift[0], shift[1], shift[2], shift[3]);
+    const dim3 grid((g.V + kSitesPerBlock - 1) / kSitesPerBlock), block(kSitesPerBlock, 4);
+    if (p == prec::f64)
+        k_shift<prec::f64><<<grid, block, 0, st>>>(static_cast<const double2*>(in),
+                                                    static_cast<double2*>(out), g, sh);
+    else
+        k_shift<prec::f32m><<<grid, block, 0, st>>>(static_cast<const float4*>(in),
+                                                     static_cast<float4*>(out), g, sh);
+    return hipGetLastError();
+}
+
 // Third row of every FP64 link rebuilt from the first two, in place: makes a
 // flow output (two stored rows) a complete nine-entry field again.
 __global__ __launch_bounds__(256) void k_complete_rows64(double2* __restrict__ U, unsigned V) {

@@ -311,6 +311,16 @@ extern "C" int qcbf_hot_start(qcbf_flow* h, uint64_t seed) {
     return QCBF_OK;
 }
 
+extern "C" int qcbf_shift(qcbf_flow* h, const int shift[4]) {
+    CHECK_H(h);
+    if (!shift) return fail(h, QCBF_ERR_ARG, "null shift");
+    int sh[4];
+    for (int d = 0; d < 4; d++) sh[d] = ((shift[d] % h->g.L[d]) + h->g.L[d]) % h->g.L[d];
+    HIPCHK(h, qcb::launch_shift(h->prec, h->U[h->cur], h->U[h->cur ^ 1], h->g, sh, h->stream));
+    h->cur ^= 1;  // the stored words move as they are; rows_stale carries over
+    return QCBF_OK;
+}
+
 extern "C" int qcbf_heatbath(qcbf_flow* h, double beta, int n_sweeps, uint64_t seed) {
     CHECK_H(h);
     if (h->prec != qcb::prec::f64)

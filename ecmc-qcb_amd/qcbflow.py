@@ -57,6 +57,7 @@ SIGNATURES = {
     "qcbf_get_links": (ctypes.c_int, [_vp, _dp]),
     "qcbf_hot_start": (ctypes.c_int, [_vp, ctypes.c_uint64]),
     "qcbf_copy_field": (ctypes.c_int, [_vp, _vp]),
+    "qcbf_shift": (ctypes.c_int, [_vp, _vp]),
     "qcbf_batch_copy_from": (ctypes.c_int, [_vp, ctypes.c_int, _vp]),
     "qcbf_cold_start": (ctypes.c_int, [_vp]),
     "qcbf_heatbath": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_uint64]),
@@ -226,6 +227,11 @@ class GradientFlow:
     def copy_from(self, other):
         """field_c <- other's field, on the device (precisions may differ)."""
         self._c(load().qcbf_copy_field(self._h, other._h))
+
+    def shift(self, s):
+        """U_mu(x) <- U_mu(x - s) on the periodic lattice (mpi::shift::shift_field)."""
+        arr = (ctypes.c_int * 4)(*[int(v) for v in s])
+        self._c(load().qcbf_shift(self._h, arr))
 
     def cold_start(self):
         self._c(load().qcbf_cold_start(self._h))

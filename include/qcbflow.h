@@ -111,6 +111,13 @@ int qcbf_cold_start(qcbf_flow* h);
  * sweep count. */
 int qcbf_heatbath(qcbf_flow* h, double beta, int n_sweeps, uint64_t seed);
 
+/* mpi::shift::shift_field (src/mpi/Shift.cpp:109-160) on the whole periodic
+ * lattice: U_mu(x) <- U_mu(x - shift) for every direction.  shift[d] is taken
+ * modulo L[d] (negative values shift the other way).  Asynchronous, like
+ * qcbf_rk3_step.  The reference's random_shift (Shift.cpp:163-186) draws the
+ * shift on the host: the compat header mpi/Shift.h does that and calls this. */
+int qcbf_shift(qcbf_flow* h, const int shift[4]);
+
 /* read_ildg_clime (src/io/ildg.cpp:138-237): load an ILDG/LIME file straight
  * into the flowing field.  path is the file itself (the reference's
  * dirpath/filename/filename composition lives in the C++ drop-in).  The

@@ -65,3 +65,38 @@ def test_heatbath_argument_checks():
         with pytest.raises(qcbflow.QcbfError) as ei:
             gf.heatbath(6.0, 1, 1)
         assert ei.value.code == qcbflow.QCBF_ERR_STATE
+
+
+def _roll(field, L, s):
+    a = field.reshape(L[3], L[2], L[1], L[0], 4, 3, 3, 2)
+    return np.roll(a, shift=(s[3], s[2], s[1], s[0]), axis=(0, 1, 2, 3)).reshape(field.shape)
+
+
+@pytest.mark.parametrize("prec", [qcbflow.FP64, qcbflow.FP32_MIXED])
+def test_device_shift_is_a_periodic_translation(prec):
+    """qcbf_shift (mpi::shift::shift_field on one device): U_mu(x) <- U_mu(x - s),
+    exactly; shifting back restores the field; a flowed field (two stored rows
+    in FP64) shifts the same way; P, E and Q do not change."""
+    L = (6, 4, 8, 10)
+    s = (2, 3, 7, 9)
+    f = oracle.hot_start(L, 9)
+    with qcbflow.GradientFlow(L, precision=prec) as gf, \
+            qcbflow.GradientFlow(L, precision=prec) as ref:
+        gf.copy(f)
+        before = gf.field()
+        m0 = gf.measure()
+        gf.shift(s)
+        assert np.array_equal(gf.field(), _roll(before, L, s))
+        m1 = gf.measure()
+        assert np.allclose(m1, m0, rtol=1e-12, atol=1e-10)
+        gf.shift([-v for v in s])
+        assert np.array_equal(gf.field(), before)
+        ref.copy(f)
+        gf.rk3_step(2)
+        ref.rk3_step(2)
+        gf.shift((13, -1, 4, 0))  # taken modulo the extents
+        assert np.array_equal(gf.field(), _roll(ref.field(), L, (1, 3, 4, 0)))
+    with qcbflow.GradientFlow(L, precision=prec) as gf:
+        with pytest.raises(qcbflow.QcbfError):
+            load = qcbflow.load()
+            gf._c(load.qcbf_shift(gf._h, None))
