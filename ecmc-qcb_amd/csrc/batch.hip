@@ -243,6 +243,7 @@ __global__ __launch_bounds__(256, 2) void k_batch_stage1_measure(
     typename storage<P>::avec* __restrict__ acc, const cfg_desc* __restrict__ cfgs,
     const unsigned* __restrict__ tile_cfg, unsigned ntiles, int map_mode, unsigned map_win,
     double cz, int* __restrict__ warn, double* __restrict__ partials) {
+    if constexpr (!kFusedMeasureBuilt) return;  // never launched (host checks)
     __shared__ typename storage<P>::real lds[kCloverSlots * 9 * 64];
     __shared__ double red[4 * 3];
     bool w = false;

@@ -114,8 +114,7 @@ __global__ __launch_bounds__(256) void k_rk3_stage1_measure(
     typename storage<P>::avec* __restrict__ acc, lattice g, double cz, int* __restrict__ warn,
     double* __restrict__ partials) {
     using FT = typename storage<P>::field;
-    static_assert(QCB_SU3_PRODUCTS && !FT::stream_rows,
-                  "the fused measurement takes its leaves from SU(3)-product staples");
+    if constexpr (!kFusedMeasureBuilt) return;  // never launched (launcher refuses)
     __shared__ typename FT::real lds[kCloverSlots * 9 * 64];
     __shared__ double red[4 * 3];
     const unsigned ntiles = measure_blocks(g);
@@ -152,7 +151,7 @@ hipError_t launch_rk3_stage1_measure(prec p, const void* Uin, void* Uout, void* 
                                      const lattice& g, double cz, int* warn, double* partials,
                                      double* out, hipStream_t st) {
     (void)hipGetLastError();  // drop a stale error of an earlier call
-    if (g.lg_tile[0] >= 0) return hipErrorInvalidValue;  // box tiles: not fusable
+    if (!kFusedMeasureBuilt || g.lg_tile[0] >= 0) return hipErrorInvalidValue;  // not fusable
     const unsigned nblk = measure_blocks(g);
     const dim3 grid(grid_for_tiles(nblk, g.map_mode, g.map_win, site_tiles_per_t(g)));
     const dim3 block(kSitesPerBlock, 4);

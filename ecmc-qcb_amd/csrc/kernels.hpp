@@ -28,6 +28,10 @@ __host__ __device__ inline unsigned measure_blocks(const lattice& g) {
 hipError_t launch_rk3_stage(prec p, int stage, const void* Uin, void* Uout, void* acc,
                             const lattice& g, double cz, double ca, int* warn, hipStream_t st);
 
+// The fused measurement hooks into the SU(3)-product staple sum: built unless
+// the library is compiled with QCB_SU3_PRODUCTS=0 or with streamed FP64 rows.
+constexpr bool kFusedMeasureBuilt = QCB_SU3_PRODUCTS && !field64::stream_rows;
+
 // RK3 stage 1 fused with the clover measurement of Uin: the stage as above,
 // plus the measurement's per-tile partials and their reduction into
 // out[0..2] = {P, E, Q}, bit-identical to launch_measure(clover = true).

@@ -617,7 +617,8 @@ extern "C" int qcbf_flow_measure(qcbf_flow* h, double eps, int n_steps_gf, int n
     double t = 0.0;
     // Every measurement but the last is followed by an RK3 step on the same
     // field: its stage 1 takes the measurement along (same results).
-    const bool fuse = h->fuse_measure && h->g.lg_tile[0] < 0 && n_rk_steps > 0;
+    const bool fuse =
+        h->fuse_measure && qcb::kFusedMeasureBuilt && h->g.lg_tile[0] < 0 && n_rk_steps > 0;
     for (int n = 0; n < n_steps_gf; n++) {
         tvals[n] = t;
         double* out = h->dres + 3 * (size_t)n;

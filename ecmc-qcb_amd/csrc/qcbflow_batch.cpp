@@ -368,7 +368,9 @@ static int enqueue_step(qcbf_batch* b, double eps, int cur, double* out = nullpt
 
 // The measurement can ride on the next stage 1 when every member uses
 // 64-consecutive-site tiles.
-static bool fusable(const qcbf_batch* b) { return b->fuse_measure && b->tile_shape == -1; }
+static bool fusable(const qcbf_batch* b) {
+    return b->fuse_measure && qcb::kFusedMeasureBuilt && b->tile_shape == -1;
+}
 
 static int enqueue_measure(qcbf_batch* b, double* out, unsigned* slot, bool bump) {
     BCHK(b, qcb::launch_batch_measure(b->prec, b->U[b->cur], b->d_cfgs, b->d_tile_cfg, b->ntiles,
