@@ -37,7 +37,20 @@ namespace qcb {
 template <prec P>
 constexpr int kStageWaves = P == prec::f64 ? 3 : 6;
 
-template <prec P, int STAGE>
+// FP64 stage kernels take the x-neighbour links of whole x-rows from lane
+// shuffles (staple_sum, XROWS): fewer vector-memory instructions for a kernel
+// whose loads keep the texture-address path about 60% busy.  The FP32-mixed
+// kernels are VALU-bound and keep their loads (the shuffled words would cost
+// them their sixth wavefront per SIMD).
+#ifndef QCB_XROW_SHUFFLES
+#define QCB_XROW_SHUFFLES 1
+#endif
+template <prec P>
+inline bool use_x_rows(const lattice& g) {
+    return QCB_XROW_SHUFFLES && P == prec::f64 && x_rows_in_wave(g);
+}
+
+template <prec P, int STAGE, bool XROWS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kStageWaves<P>))) void k_rk3_stage(
     const typename storage<P>::vec* __restrict__ Uin, typename storage<P>::vec* __restrict__ Uout,
     typename storage<P>::avec* __restrict__ acc, lattice g, double cz, double ca,
@@ -48,15 +61,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kStageWaves
     if (tile >= ntiles) return;
     unsigned c[4];
     const unsigned s = tile_site(g, tile, threadIdx.x, c);
-    if (s >= g.V) return;
-    const nbr n = neighbours_at(g, c);
+    if (s >= g.V) return;  // never a partial wavefront when XROWS (full tiles)
+    nbr n = neighbours_at(g, c);
+    if constexpr (XROWS) set_x_lanes(n, g, threadIdx.x);
     const auto f = storage<P>::view(Uin, g.V);
     bool w = false;
     switch (threadIdx.y) {
-        case 0: stage_body<P, STAGE, 0>(f, Uout, acc, s, n, cz, ca, w); break;
-        case 1: stage_body<P, STAGE, 1>(f, Uout, acc, s, n, cz, ca, w); break;
-        case 2: stage_body<P, STAGE, 2>(f, Uout, acc, s, n, cz, ca, w); break;
-        default: stage_body<P, STAGE, 3>(f, Uout, acc, s, n, cz, ca, w); break;
+        case 0: stage_body<P, STAGE, 0, XROWS>(f, Uout, acc, s, n, cz, ca, w); break;
+        case 1: stage_body<P, STAGE, 1, XROWS>(f, Uout, acc, s, n, cz, ca, w); break;
+        case 2: stage_body<P, STAGE, 2, XROWS>(f, Uout, acc, s, n, cz, ca, w); break;
+        default: stage_body<P, STAGE, 3, XROWS>(f, Uout, acc, s, n, cz, ca, w); break;
     }
     if (w) atomicOr(warn, 1);
 }
@@ -70,10 +84,14 @@ static hipError_t stage_launch(int stage, const void* Uin, void* Uout, void* acc
     auto* a = static_cast<typename S::avec*>(acc);
     const dim3 block(kSitesPerBlock, 4);
     const dim3 grid(grid_for_tiles(stage_tiles(g), g.map_mode, g.map_win, stage_tiles_per_t(g)));
-    switch (stage) {
-        case 1: k_rk3_stage<P, 1><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
-        case 2: k_rk3_stage<P, 2><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
-        case 3: k_rk3_stage<P, 3><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
+    const bool xr = use_x_rows<P>(g);
+    switch (stage * 2 + (xr ? 1 : 0)) {
+        case 2: k_rk3_stage<P, 1, false><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
+        case 3: k_rk3_stage<P, 1, true><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
+        case 4: k_rk3_stage<P, 2, false><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
+        case 5: k_rk3_stage<P, 2, true><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
+        case 6: k_rk3_stage<P, 3, false><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
+        case 7: k_rk3_stage<P, 3, true><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -96,19 +114,19 @@ hipError_t launch_rk3_stage(prec p, int stage, const void* Uin, void* Uout, void
 // instead of a separate measurement.  Its per-tile partials are those of
 // k_measure_clover (same per-site code, same workgroup reduction).  Stage
 // tiles must be 64 consecutive sites (the default shape).
-template <prec P, int MU>
+template <prec P, int MU, bool XROWS>
 __device__ inline double stage1_measure_wave(const typename storage<P>::field& f,
                                              typename storage<P>::vec* __restrict__ Uout,
                                              typename storage<P>::avec* __restrict__ acc,
                                              unsigned s, const nbr& n, double cz, bool& warn,
                                              typename storage<P>::real* lds, unsigned lane) {
     clover_parts<typename storage<P>::field, MU> h{f, s, n, f.link(MU, s), lds, lane, 0.0, {}};
-    stage_body<P, 1, MU>(f, Uout, acc, s, n, cz, 0.0, warn, h);
+    stage_body<P, 1, MU, XROWS>(f, Uout, acc, s, n, cz, 0.0, warn, h);
     h.finish();
     return h.plaq;
 }
 
-template <prec P>
+template <prec P, bool XROWS>
 __global__ __launch_bounds__(256) void k_rk3_stage1_measure(
     const typename storage<P>::vec* __restrict__ Uin, typename storage<P>::vec* __restrict__ Uout,
     typename storage<P>::avec* __restrict__ acc, lattice g, double cz, int* __restrict__ warn,
@@ -125,15 +143,16 @@ __global__ __launch_bounds__(256) void k_rk3_stage1_measure(
     const unsigned s = tile * kSitesPerBlock + lane;
     const bool valid = s < g.V;
     const FT f = storage<P>::view(Uin, g.V);
-    const nbr n = neighbours(g, valid ? s : 0u);
+    nbr n = neighbours(g, valid ? s : 0u);
+    if constexpr (XROWS) set_x_lanes(n, g, lane);  // full tiles: every lane is valid
     double v[3] = {0.0, 0.0, 0.0};
     bool w = false;
     if (valid) {
         switch (threadIdx.y) {
-            case 0: v[0] = stage1_measure_wave<P, 0>(f, Uout, acc, s, n, cz, w, lds, lane); break;
-            case 1: v[0] = stage1_measure_wave<P, 1>(f, Uout, acc, s, n, cz, w, lds, lane); break;
-            case 2: v[0] = stage1_measure_wave<P, 2>(f, Uout, acc, s, n, cz, w, lds, lane); break;
-            default: v[0] = stage1_measure_wave<P, 3>(f, Uout, acc, s, n, cz, w, lds, lane); break;
+            case 0: v[0] = stage1_measure_wave<P, 0, XROWS>(f, Uout, acc, s, n, cz, w, lds, lane); break;
+            case 1: v[0] = stage1_measure_wave<P, 1, XROWS>(f, Uout, acc, s, n, cz, w, lds, lane); break;
+            case 2: v[0] = stage1_measure_wave<P, 2, XROWS>(f, Uout, acc, s, n, cz, w, lds, lane); break;
+            default: v[0] = stage1_measure_wave<P, 3, XROWS>(f, Uout, acc, s, n, cz, w, lds, lane); break;
         }
     }
     __syncthreads();
@@ -155,14 +174,18 @@ hipError_t launch_rk3_stage1_measure(prec p, const void* Uin, void* Uout, void* 
     const unsigned nblk = measure_blocks(g);
     const dim3 grid(grid_for_tiles(nblk, g.map_mode, g.map_win, site_tiles_per_t(g)));
     const dim3 block(kSitesPerBlock, 4);
-    if (p == prec::f64)
-        k_rk3_stage1_measure<prec::f64><<<grid, block, 0, st>>>(
-            static_cast<const double2*>(Uin), static_cast<double2*>(Uout),
-            static_cast<double2*>(acc), g, cz, warn, partials);
+    const auto* i64 = static_cast<const double2*>(Uin);
+    auto* o64 = static_cast<double2*>(Uout);
+    auto* a64 = static_cast<double2*>(acc);
+    const auto* i32 = static_cast<const float4*>(Uin);
+    auto* o32 = static_cast<float4*>(Uout);
+    auto* a32 = static_cast<float4*>(acc);
+    if (p == prec::f64 && use_x_rows<prec::f64>(g))
+        k_rk3_stage1_measure<prec::f64, true><<<grid, block, 0, st>>>(i64, o64, a64, g, cz, warn, partials);
+    else if (p == prec::f64)
+        k_rk3_stage1_measure<prec::f64, false><<<grid, block, 0, st>>>(i64, o64, a64, g, cz, warn, partials);
     else
-        k_rk3_stage1_measure<prec::f32m><<<grid, block, 0, st>>>(
-            static_cast<const float4*>(Uin), static_cast<float4*>(Uout), static_cast<float4*>(acc),
-            g, cz, warn, partials);
+        k_rk3_stage1_measure<prec::f32m, false><<<grid, block, 0, st>>>(i32, o32, a32, g, cz, warn, partials);
     const hipError_t e = hipGetLastError();
     return e != hipSuccess ? e : launch_reduce_measure(partials, g, out, st);
 }

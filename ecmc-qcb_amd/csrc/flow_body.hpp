@@ -29,6 +29,24 @@ __device__ inline mat3<typename F::real> lower_staple(const F& f, unsigned s, co
     return mul_dn_su3(mul_su3(f.link(MU, xmnu), f.link(nu, xmunu)), f.link(nu, xmnu));
 }
 
+// The link words of lane src (the words a load at that lane's site returns).
+template <class F>
+__device__ inline void shfl_words(const typename F::word (&w)[F::kWords], int src,
+                                  typename F::word (&out)[F::kWords]) {
+#pragma unroll
+    for (int k = 0; k < F::kWords; k++) {
+        if constexpr (sizeof(typename F::word) == 16 && sizeof(typename F::real) == 8) {
+            out[k].x = __shfl(w[k].x, src, 64);
+            out[k].y = __shfl(w[k].y, src, 64);
+        } else {
+            out[k].x = __shfl(w[k].x, src, 64);
+            out[k].y = __shfl(w[k].y, src, 64);
+            out[k].z = __shfl(w[k].z, src, 64);
+            out[k].w = __shfl(w[k].w, src, 64);
+        }
+    }
+}
+
 // Receives the two staples of every plane as the staple sum forms them, the
 // upper one first (the fused measurement builds three clover leaves from
 // them); the default does nothing.
@@ -46,8 +64,10 @@ struct no_staple_hook {
 // Without them, for FP64 one operand of every product is streamed from memory
 // a row at a time, so only the accumulator, one product and one full operand
 // are live; FP32 matrices are half the registers and are loaded whole.  The
-// hook is called only on the SU(3)-product path.
-template <class F, int MU, class Hook = no_staple_hook>
+// hook is called only on the SU(3)-product path.  XROWS: the wavefront holds
+// whole x-rows (x_rows_in_wave, lattice.hpp; n.xp and n.xm set), so the links
+// of the x-neighbours come from lane shuffles.
+template <class F, int MU, bool XROWS = false, class Hook = no_staple_hook>
 __device__ inline mat3<typename F::real> staple_sum(const F& f, unsigned s, const nbr& n,
                                                     Hook&& hook = Hook{}) {
     using T = typename F::real;
@@ -105,18 +125,52 @@ __device__ inline mat3<typename F::real> staple_sum(const F& f, unsigned s, cons
             }
         } else if constexpr (QCB_SU3_PRODUCTS) {
             // every product of links is in SU(3): form two rows, rebuild the third
-            {
-                const mat3<T> wu = upper_staple<F, MU>(f, s, n, nu);
-                add_to(st, wu);
-                hook.upper(nu, wu);
-            }
+            if (XROWS && (MU == 0 || nu == 0)) {
+                // The x-neighbours of this plane's links are in this wavefront:
+                // shuffle them instead of loading them (same words, same
+                // products as upper_staple / lower_staple).
+                using W = typename F::word;
+                if (MU == 0) {  // U_nu(x+mu) is lane xp's U_nu(x)
+                    W a[F::kWords], ap[F::kWords];
+                    f.load_words(nu, s, a);
+                    shfl_words<F>(a, n.xp, ap);
+                    const mat3<T> wu = mul_nd_su3(F::from_words(ap),
+                                                  mul_su3(F::from_words(a), f.link(MU, xnu)));
+                    add_to(st, wu);
+                    hook.upper(nu, wu);
+                    const mat3<T> wd = lower_staple<F, MU>(f, s, n, nu);
+                    add_to(st, wd);
+                    hook.lower(nu, wd);
+                } else {  // nu == 0: U_mu(x+nu), U_mu(x-nu), U_nu(x-nu) from lanes
+                    W own[F::kWords], a[F::kWords], t[F::kWords];
+                    f.load_words(MU, s, own);
+                    f.load_words(nu, s, a);
+                    shfl_words<F>(own, n.xp, t);
+                    const mat3<T> wu = mul_nd_su3(f.link(nu, xmu),
+                                                  mul_su3(F::from_words(a), F::from_words(t)));
+                    add_to(st, wu);
+                    hook.upper(nu, wu);
+                    shfl_words<F>(own, n.xm, t);
+                    const mat3<T> b = mul_su3(F::from_words(t), f.link(nu, xmunu));
+                    shfl_words<F>(a, n.xm, t);
+                    const mat3<T> wd = mul_dn_su3(b, F::from_words(t));
+                    add_to(st, wd);
+                    hook.lower(nu, wd);
+                }
+            } else {
+                {
+                    const mat3<T> wu = upper_staple<F, MU>(f, s, n, nu);
+                    add_to(st, wu);
+                    hook.upper(nu, wu);
+                }
 #if QCB_SCHED_SPLIT >= 2
-            if constexpr (sizeof(T) == 8) __builtin_amdgcn_sched_barrier(0);
+                if constexpr (sizeof(T) == 8) __builtin_amdgcn_sched_barrier(0);
 #endif
-            {
-                const mat3<T> wd = lower_staple<F, MU>(f, s, n, nu);
-                add_to(st, wd);
-                hook.lower(nu, wd);
+                {
+                    const mat3<T> wd = lower_staple<F, MU>(f, s, n, nu);
+                    add_to(st, wd);
+                    hook.lower(nu, wd);
+                }
             }
         } else {
             {
@@ -140,16 +194,16 @@ __device__ inline mat3<typename F::real> staple_sum(const F& f, unsigned s, cons
     return st;
 }
 
-template <class F, int MU, class Hook = no_staple_hook>
+template <class F, int MU, bool XROWS = false, class Hook = no_staple_hook>
 __device__ inline alg8<typename F::real> force_at(const F& f, unsigned s, const nbr& n,
                                                   mat3<typename F::real>& u,
                                                   Hook&& hook = Hook{}) {
-    const mat3<typename F::real> st = staple_sum<F, MU>(f, s, n, hook);
+    const mat3<typename F::real> st = staple_sum<F, MU, XROWS>(f, s, n, hook);
     u = f.link(MU, s);
     return neg_proj_ta(mul(u, st));
 }
 
-template <prec P, int STAGE, int MU, class Hook = no_staple_hook>
+template <prec P, int STAGE, int MU, bool XROWS = false, class Hook = no_staple_hook>
 __device__ inline void stage_body(const typename storage<P>::field& f,
                                   typename storage<P>::vec* __restrict__ Uout,
                                   typename storage<P>::avec* __restrict__ acc, unsigned s,
@@ -158,7 +212,7 @@ __device__ inline void stage_body(const typename storage<P>::field& f,
     using S = storage<P>;
     using T = typename S::real;
     mat3<T> u;
-    const alg8<T> z = force_at<typename S::field, MU>(f, s, n, u, hook);
+    const alg8<T> z = force_at<typename S::field, MU, XROWS>(f, s, n, u, hook);
     const T czr = (T)cz, car = (T)ca;
     alg8<T> x;
     if constexpr (STAGE == 1) {

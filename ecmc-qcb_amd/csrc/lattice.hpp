@@ -160,7 +160,23 @@ inline lattice make_lattice(const int dims[4]) {
 struct nbr {
     int up[4];
     int dn[4];
+    // Lanes holding the sites x+1 and x-1 when the wavefront's 64 sites are
+    // whole x-rows (set_x_lanes); -1 otherwise.
+    int xp = -1, xm = -1;
 };
+
+// A wavefront of a 64-consecutive-site tile holds whole x-rows when Lx divides
+// 64 and every tile is full: the links at x+1 and x-1 are then in the same
+// wavefront, and the staple sum can take them from a lane shuffle instead of
+// a load (staple_sum, flow_body.hpp).  All 64 lanes must take part.
+__host__ __device__ inline bool x_rows_in_wave(const lattice& g) {
+    return g.lg_tile[0] < 0 && g.L[0] <= 64 && 64 % g.L[0] == 0 && g.V % 64 == 0;
+}
+__device__ inline void set_x_lanes(nbr& n, const lattice& g, unsigned lane) {
+    const unsigned Lx = (unsigned)g.L[0], x = lane & (Lx - 1), base = lane - x;
+    n.xp = (int)(base + ((x + 1) & (Lx - 1)));
+    n.xm = (int)(base + ((x + Lx - 1) & (Lx - 1)));
+}
 
 __device__ inline nbr neighbours_at(const lattice& g, const unsigned (&c)[4]) {
     nbr n;
