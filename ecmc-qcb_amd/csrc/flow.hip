@@ -85,13 +85,14 @@ static hipError_t stage_launch(int stage, const void* Uin, void* Uout, void* acc
     const dim3 block(kSitesPerBlock, 4);
     const dim3 grid(grid_for_tiles(stage_tiles(g), g.map_mode, g.map_win, stage_tiles_per_t(g)));
     const bool xr = use_x_rows<P>(g);
+    constexpr bool XR = P == prec::f64;  // only FP64 has x-row variants
     switch (stage * 2 + (xr ? 1 : 0)) {
         case 2: k_rk3_stage<P, 1, false><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
-        case 3: k_rk3_stage<P, 1, true><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
+        case 3: k_rk3_stage<P, 1, XR><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
         case 4: k_rk3_stage<P, 2, false><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
-        case 5: k_rk3_stage<P, 2, true><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
+        case 5: k_rk3_stage<P, 2, XR><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
         case 6: k_rk3_stage<P, 3, false><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
-        case 7: k_rk3_stage<P, 3, true><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
+        case 7: k_rk3_stage<P, 3, XR><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
