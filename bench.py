@@ -378,7 +378,7 @@ def main():
     ap.add_argument("--sweeps", type=int, default=None,
                     help="heatbath sweeps of the input (default 50)")
     ap.add_argument("--report", choices=("precision", "ensemble", "large"), default=None)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=12)  # about 11 s on 16 host threads
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-mixed leg")
