@@ -209,6 +209,20 @@ def volume(L):
     return V
 
 
+# Device bytes per site a single-lattice handle keeps resident: two link
+# fields (ping-pong) and the 8-real su(3) accumulator.  An FP32-mixed link is
+# two rows (12 FP32 reals, 48 B); the 18-real FP32 form would take 72 B.
+LINK_FIELD_BYTES_PER_SITE = {"fp64": 4 * 144, "fp32_mixed": 4 * 48}
+ACC_BYTES_PER_SITE = {"fp64": 4 * 64, "fp32_mixed": 4 * 32}
+
+
+def device_memory(V, tag):
+    link = LINK_FIELD_BYTES_PER_SITE[tag] * V
+    acc = ACC_BYTES_PER_SITE[tag] * V
+    return {"link_field_gb": link / 1e9, "resident_gb": (2 * link + acc) / 1e9,
+            "fp32_18_real_field_gb": 4 * 72 * V / 1e9 if tag == "fp32_mixed" else None}
+
+
 def report_single(qcbflow, args, name, L, beta, seed, sweeps, K, every):
     """FP64 and FP32-mixed flows of one thermalised lattice: throughput, stage
     roofline, and the FP32-mixed errors against FP64 (P, E, Q, t0)."""
@@ -237,6 +251,7 @@ def report_single(qcbflow, args, name, L, beta, seed, sweeps, K, every):
             "stage1_measure": fused_stats(ms, n, V, tag),
             "launch_shape": shape,
             "exp_warnings": h.exp_warnings(),
+            "device_memory": device_memory(V, tag),
         }
         series[tag] = (tqe, plaq)
         log(f"  {tag}: {legs[tag]['value']:.4e} site-steps/s, stage {avg:.4f} ms "
