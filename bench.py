@@ -257,6 +257,9 @@ def report_single(qcbflow, args, name, L, beta, seed, sweeps, K, every):
         log(f"  {tag}: {legs[tag]['value']:.4e} site-steps/s, stage {avg:.4f} ms "
             f"({gbs:.0f} GB/s), E(t={tqe[-1, 0]:.2f}) = {tqe[-1, 2]:.6g}, Q = {tqe[-1, 1]:.6g}")
         h.close()
+    copy_ms = None
+    if args.calibrate_copy_mib > 0:
+        copy_ms = src.copy_bench(args.calibrate_copy_mib << 20, 10)
     src.close()
     t64, p64 = series["fp64"]
     t32, p32 = series["fp32_mixed"]
@@ -285,6 +288,8 @@ def report_single(qcbflow, args, name, L, beta, seed, sweeps, K, every):
         "fp32_mixed_vs_fp64": legs["fp32_mixed"]["value"] / legs["fp64"]["value"],
         "accuracy_fp32_mixed_vs_fp64": acc,
         "data": "synthetic: GPU Haar hot start + GPU heatbath",
+        "copy_calibration": None if copy_ms is None else {"mib": args.calibrate_copy_mib,
+                                                          "avg_ms": copy_ms},
     }
 
 

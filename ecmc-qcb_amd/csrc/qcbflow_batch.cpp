@@ -45,7 +45,7 @@ struct qcbf_batch {
     int* dwarn = nullptr;
     void* stage = nullptr;
     qcb::batch_launch bl{0, 2, 1024};
-    int blocks_per_cu = 2;
+    int blocks_per_cu = 0;  // one workgroup per tile: the autotuner's pick on MI355X
     long tile_shape = -1;
     int cus = 256;
     bool use_graph = false;

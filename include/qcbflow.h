@@ -190,7 +190,7 @@ enum qcbf_tune_key {
     QCBF_TUNE_MAP_WIN = 1,  /* mode 2: window (rounded down to a multiple of 8, at
                                least 8); mode 3: slab width in tiles; >= 1 */
     QCBF_TUNE_BLOCKS_PER_CU = 2, /* ensembles only: persistent-grid workgroups per CU
-                                    (1..16), or 0 for one workgroup per tile */
+                                    (1..16), or 0 (default) for one workgroup per tile */
     QCBF_TUNE_FUSED_MEASURE = 4, /* qcbf_flow_measure fuses each measurement into the
                                     next RK3 stage 1: 1 (default) or 0 (separate
                                     kernels); results are bit-identical either way.
