@@ -33,14 +33,13 @@ __device__ inline mat3<typename F::real> lower_staple(const F& f, unsigned s, co
 template <class F>
 __device__ inline void shfl_words(const typename F::word (&w)[F::kWords], int src,
                                   typename F::word (&out)[F::kWords]) {
+    // a word is a double2 (FP64) or a float4 (FP32 two-row form)
+    constexpr bool four = sizeof(typename F::word) == 4 * sizeof(typename F::real);
 #pragma unroll
     for (int k = 0; k < F::kWords; k++) {
-        if constexpr (sizeof(typename F::word) == 16 && sizeof(typename F::real) == 8) {
-            out[k].x = __shfl(w[k].x, src, 64);
-            out[k].y = __shfl(w[k].y, src, 64);
-        } else {
-            out[k].x = __shfl(w[k].x, src, 64);
-            out[k].y = __shfl(w[k].y, src, 64);
+        out[k].x = __shfl(w[k].x, src, 64);
+        out[k].y = __shfl(w[k].y, src, 64);
+        if constexpr (four) {
             out[k].z = __shfl(w[k].z, src, 64);
             out[k].w = __shfl(w[k].w, src, 64);
         }
