@@ -36,7 +36,12 @@ namespace qcb {
 // and keeping the staple sum in LDS (130 VGPRs at 3 waves, 128 + spills at 4).
 // Four waves with loaded x-neighbours (128 VGPRs, at most 8 spilled, a
 // scheduling barrier between the two staples of a plane) ran 7-12% slower
-// than three waves with lane shuffles at 32^4 and 64^4.
+// than three waves with lane shuffles at 32^4 and 64^4.  Forming each
+// plaquette once and handing -P_TA of it to the other direction's wavefront
+// through LDS (U_mu(x) times the upper staple of plane (mu, nu) is the
+// plaquette, the adjoint of wavefront nu's term) cut the VALU instructions
+// by 7-8% but, with its barrier, was no faster than the lane shuffles in FP64
+// and 5-13% slower in FP32-mixed.
 template <prec P>
 constexpr int kStageWaves = P == prec::f64 ? 3 : 6;
 
