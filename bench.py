@@ -182,13 +182,14 @@ def fused_stats(ms, launches, V, tag):
 
 
 def setup_shape(qcbflow, h, args):
-    """launch shape: explicit knobs, then (unless disabled) the autotuner"""
+    """launch shape: the library's default (slab sweep over 64-site tiles),
+    explicit knobs, then the autotuner if asked for"""
     for key, val in ((qcbflow.TUNE_MAP_MODE, args.map_mode),
                      (qcbflow.TUNE_MAP_WIN, args.map_win),
                      (qcbflow.TUNE_TILE_SHAPE, args.tile_shape)):
         if val is not None:
             h.tune(key, val)
-    if not args.no_autotune:
+    if args.autotune:
         h.autotune(trial_steps=2)
     if args.no_fuse:
         h.tune(qcbflow.TUNE_FUSED_MEASURE, 0)
@@ -350,7 +351,7 @@ def report_ensemble(qcbflow, args):
         if args.no_fuse:
             e.tune(qcbflow.TUNE_FUSED_MEASURE, 0)
         m, w, b = (None, None, None)
-        if not args.no_autotune:
+        if not args.no_autotune:  # the ensemble configuration is autotuned
             m, w, b = e.autotune(trial_steps=2)
         e.synchronize()
         t0 = time.perf_counter()
@@ -406,8 +407,11 @@ def main():
                     help="workgroup->tile order (qcbf_tune); default: the library's")
     ap.add_argument("--map-win", type=int, default=None)
     ap.add_argument("--tile-shape", type=int, default=None, help="qcbf_tune tile-shape code")
+    ap.add_argument("--autotune", action="store_true",
+                    help="single lattice: run qcbf_autotune before the warmup (default: the "
+                         "library's launch shape, which the autotuner matches within noise)")
     ap.add_argument("--no-autotune", action="store_true",
-                    help="skip qcbf_autotune before the warmup")
+                    help="ensemble report: skip qcbf_batch_autotune")
     ap.add_argument("--no-fuse", action="store_true",
                     help="measure with a separate kernel instead of inside the next RK3 stage 1")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -487,11 +491,12 @@ def main():
     log(f"host transfers of the field: up {t_up * 1e3:.1f} ms, down {t_down * 1e3:.1f} ms -> "
         f"{pcie_value:.4e} site-steps/s including them")
 
-    copy_ms = None
-    if args.calibrate_copy_mib > 0:
-        copy_ms = gf.copy_bench(args.calibrate_copy_mib << 20, 10)
-        log(f"copy kernel: {args.calibrate_copy_mib} MiB in {copy_ms:.4f} ms -> "
-            f"{2 * (args.calibrate_copy_mib << 20) / (copy_ms * 1e-3) / 1e9:.1f} GB/s (read+write)")
+    # a plain device copy of 1 GiB (or the calibration size): the bandwidth a
+    # simple streaming kernel reaches on this GPU, beside the published peak
+    copy_mib = args.calibrate_copy_mib if args.calibrate_copy_mib > 0 else 1024
+    copy_ms = gf.copy_bench(copy_mib << 20, 10)
+    copy_gbs = 2 * (copy_mib << 20) / (copy_ms * 1e-3) / 1e9
+    log(f"copy kernel: {copy_mib} MiB in {copy_ms:.4f} ms -> {copy_gbs:.1f} GB/s (read+write)")
     gf.close()
 
     # the FP32-mixed form on the same input and cadence
@@ -568,6 +573,8 @@ def main():
             "avg_launch_ms": avg_stage_ms,
             "fp64_tflops": flops_rate,
             "fp64_frac": flops_rate / FP64_PEAK_TFLOPS,
+            "copy_kernel_gbs": copy_gbs,
+            "traffic_gbs": None if traffic is None else traffic / (avg_stage_ms * 1e-3) / 1e9,
             **per_stage,
         },
         "measure_avg_ms": meas_avg_ms,
@@ -578,8 +585,7 @@ def main():
         "fp32_mixed": mixed,
         "cpu_baseline": cpu,
     }
-    if copy_ms is not None:
-        out["copy_calibration"] = {"mib": args.calibrate_copy_mib, "avg_ms": copy_ms}
+    out["copy_calibration"] = {"mib": copy_mib, "avg_ms": copy_ms}
     print(json.dumps(out), flush=True)
 
 
