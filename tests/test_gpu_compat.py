@@ -3,6 +3,7 @@ read_ildg_clime signatures over libqcbflow) driven by a caller shaped like the
 reference's src/bin/test.cpp, against the committed oracle trajectory."""
 import json
 import os
+import re
 import shutil
 import subprocess
 
@@ -13,6 +14,7 @@ from conftest import ROOT
 
 pytestmark = pytest.mark.gpu
 EXE = os.path.join(ROOT, "ecmc-qcb_amd", "compat", "flow_measure")
+STAGES = os.path.join(ROOT, "ecmc-qcb_amd", "compat", "flow_stages")
 
 
 def test_compat_caller_matches_golden(golden_dir, tmp_path):
@@ -46,3 +48,31 @@ def test_compat_missing_file_exits_with_status_1(tmp_path):
     out = subprocess.run([EXE, str(tmp_path), "missing", "4", "2", "1"], capture_output=True,
                          text=True, timeout=60)
     assert out.returncode == 1 and "missing" in out.stderr
+
+
+def test_compat_stage_members_match_rk3_step_and_oracle(golden_dir, tmp_path):
+    """GradientFlow's members called one by one in rk3_step's order
+    (src/flow/gradient_flow.cpp:124-135) through the drop-in, with the host
+    mirrors force_0 and field_c, against rk3_step and against the oracle."""
+    import limefile
+    import oracle
+
+    print("compat flow_stages on the 4^4 fixture", flush=True)
+    conf = os.path.join(golden_dir, "conf_4x4x4x4.ildg")
+    d = tmp_path / "conf"
+    d.mkdir()
+    shutil.copy(conf, d / "conf")
+    out = subprocess.run([STAGES, str(tmp_path), "conf", "4", str(tmp_path)], capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    dmax = float(re.search(r"stage sequence\| = (\S+)", out.stdout).group(1))
+    assert dmax < 1e-13  # fused stage kernels against the unfused members
+    assert "differing from their periodic image: 0" in out.stdout
+    L = (4, 4, 4, 4)
+    f = np.ascontiguousarray(limefile.read_ildg(conf))
+    force0 = np.fromfile(tmp_path / "force0.bin")
+    assert np.max(np.abs(force0 - oracle.compute_force(f.copy(), L))) < 1e-13
+    rk3 = np.fromfile(tmp_path / "rk3.bin")
+    assert np.max(np.abs(rk3 - oracle.rk3(f.copy(), L, 0.02, 1))) < 1e-13
+    stepwise = np.fromfile(tmp_path / "stepwise.bin")
+    assert np.max(np.abs(stepwise - oracle.rk3(f.copy(), L, 0.02, 1))) < 1e-13
