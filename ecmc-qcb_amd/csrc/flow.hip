@@ -177,7 +177,7 @@ __global__ __launch_bounds__(256) void k_rk3_stage1_measure(
 
 hipError_t launch_rk3_stage1_measure(prec p, const void* Uin, void* Uout, void* acc,
                                      const lattice& g, double cz, int* warn, double* partials,
-                                     hipStream_t st) {
+                                     double* out, hipStream_t st) {
     (void)hipGetLastError();  // drop a stale error of an earlier call
     if (!kFusedMeasureBuilt || g.lg_tile[0] >= 0) return hipErrorInvalidValue;  // not fusable
     const unsigned nblk = measure_blocks(g);
@@ -195,7 +195,8 @@ hipError_t launch_rk3_stage1_measure(prec p, const void* Uin, void* Uout, void* 
         k_rk3_stage1_measure<prec::f64, false><<<grid, block, 0, st>>>(i64, o64, a64, g, cz, warn, partials);
     else
         k_rk3_stage1_measure<prec::f32m, false><<<grid, block, 0, st>>>(i32, o32, a32, g, cz, warn, partials);
-    return hipGetLastError();
+    const hipError_t e = hipGetLastError();
+    return e != hipSuccess ? e : launch_reduce_measure(partials, g, out, st);
 }
 
 // ---- unfused pieces (GradientFlow member functions) -------------------------

@@ -33,13 +33,12 @@ hipError_t launch_rk3_stage(prec p, int stage, const void* Uin, void* Uout, void
 constexpr bool kFusedMeasureBuilt = QCB_SU3_PRODUCTS && !field64::stream_rows;
 
 // RK3 stage 1 fused with the clover measurement of Uin: the stage as above,
-// plus the measurement's per-tile partials (measure_blocks(g) triples);
-// launch_reduce_measure on them gives out[0..2] = {P, E, Q}, bit-identical
-// to launch_measure(clover = true).  Needs the default stage tiles (64
-// consecutive sites).
+// plus the measurement's per-tile partials and their reduction into
+// out[0..2] = {P, E, Q}, bit-identical to launch_measure(clover = true).
+// Needs the default stage tiles (64 consecutive sites).
 hipError_t launch_rk3_stage1_measure(prec p, const void* Uin, void* Uout, void* acc,
                                      const lattice& g, double cz, int* warn, double* partials,
-                                     hipStream_t st);
+                                     double* out, hipStream_t st);
 
 // Unfused pieces, one per GradientFlow member function.
 hipError_t launch_force(prec p, const void* U, void* F, const lattice& g, hipStream_t st);

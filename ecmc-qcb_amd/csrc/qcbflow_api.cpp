@@ -33,13 +33,6 @@ struct qcbf_flow {
     qcb::prec prec = qcb::prec::f64;
     lattice g{};
     hipStream_t stream = nullptr;
-    // Side stream of the fused measurement's reduction: it runs beside RK3
-    // stages 2 and 3, which do not need its result.  ev_part (main stream)
-    // orders it after the partials are written, ev_red (side stream) orders
-    // the next writer of the partials, and any reader of dres, after it.
-    hipStream_t side = nullptr;
-    hipEvent_t ev_part = nullptr, ev_red = nullptr;
-    bool red_pending = false;
     void* U[2] = {nullptr, nullptr};  // ping-pong link fields
     int cur = 0;                      // U[cur] is the flowing field
     bool rows_stale = false;          // U[cur] holds rows 0-1 only (a flow output)
@@ -130,7 +123,6 @@ static int timing_end(qcbf_flow* h, qcbf_event* ev) {
 // ---- lifetime -----------------------------------------------------------------
 static void release(qcbf_flow* h) {
     (void)hipSetDevice(h->dev);
-    if (h->side) (void)hipStreamSynchronize(h->side);
     for (auto& e : h->events) {
         if (e.a) (void)hipEventDestroy(e.a);
         if (e.b) (void)hipEventDestroy(e.b);
@@ -144,9 +136,6 @@ static void release(qcbf_flow* h) {
     (void)hipFree(h->dres);
     (void)hipFree(h->dwarn);
     (void)hipFree(h->stage);
-    if (h->ev_part) (void)hipEventDestroy(h->ev_part);
-    if (h->ev_red) (void)hipEventDestroy(h->ev_red);
-    if (h->side) (void)hipStreamDestroy(h->side);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -155,19 +144,8 @@ extern "C" const char* qcbf_last_error(const qcbf_flow* h) {
     return h ? h->err.c_str() : g_err.c_str();
 }
 
-// Orders the main stream after a reduction still running on the side stream
-// (before the partials are written again or dres is read).
-static int wait_reduction(qcbf_flow* h) {
-    if (!h->red_pending) return QCBF_OK;
-    HIPCHK(h, hipStreamWaitEvent(h->stream, h->ev_red, 0));
-    h->red_pending = false;
-    return QCBF_OK;
-}
-
 static int ensure_results(qcbf_flow* h, size_t ndoubles) {
     if (ndoubles <= h->dres_cap) return QCBF_OK;
-    HIPCHK(h, hipStreamSynchronize(h->side));  // no reduction may still write dres
-    h->red_pending = false;
     HIPCHK(h, hipStreamSynchronize(h->stream));
     (void)hipFree(h->dres);
     h->dres = nullptr;
@@ -209,9 +187,6 @@ extern "C" int qcbf_create(const int dims[4], int precision, qcbf_flow** out) {
         }                                                                                     \
     } while (0)
     CREATE_CHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-    CREATE_CHK(hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
-    CREATE_CHK(hipEventCreateWithFlags(&h->ev_part, hipEventDisableTiming));
-    CREATE_CHK(hipEventCreateWithFlags(&h->ev_red, hipEventDisableTiming));
     CREATE_CHK(hipMalloc(&h->U[0], qcb::link_field_bytes(h->prec, h->g.V)));
     CREATE_CHK(hipMalloc(&h->U[1], qcb::link_field_bytes(h->prec, h->g.V)));
     CREATE_CHK(hipMalloc(&h->acc, qcb::alg_field_bytes(h->prec, h->g.V)));
@@ -496,27 +471,18 @@ static int rk3_once(qcbf_flow* h, double eps, double* measure_out = nullptr) {
     for (int stage = 1; stage <= 3; stage++) {
         qcbf_event* ev;
         const bool fused = stage == 1 && measure_out;
-        int rc = fused ? wait_reduction(h) : QCBF_OK;  // the partials are rewritten
-        if (rc || (rc = timing_begin(h, fused ? QCBF_K_STAGE1_MEASURE : QCBF_K_STAGE1 + stage - 1,
-                                     &ev)))
-            return rc;
+        int rc = timing_begin(h, fused ? QCBF_K_STAGE1_MEASURE : QCBF_K_STAGE1 + stage - 1, &ev);
+        if (rc) return rc;
         if (fused)
             HIPCHK(h, qcb::launch_rk3_stage1_measure(h->prec, h->U[h->cur], h->U[h->cur ^ 1],
                                                      h->acc, h->g, cz[0], h->dwarn, h->partials,
-                                                     h->stream));
+                                                     measure_out, h->stream));
         else
             HIPCHK(h, qcb::launch_rk3_stage(h->prec, stage, h->U[h->cur], h->U[h->cur ^ 1],
                                             h->acc, h->g, cz[stage - 1], ca[stage - 1], h->dwarn,
                                             h->stream));
         rc = timing_end(h, ev);
         if (rc) return rc;
-        if (fused) {  // the reduction into measure_out runs beside stages 2 and 3
-            HIPCHK(h, hipEventRecord(h->ev_part, h->stream));
-            HIPCHK(h, hipStreamWaitEvent(h->side, h->ev_part, 0));
-            HIPCHK(h, qcb::launch_reduce_measure(h->partials, h->g, measure_out, h->side));
-            HIPCHK(h, hipEventRecord(h->ev_red, h->side));
-            h->red_pending = true;
-        }
         h->cur ^= 1;
     }
     h->rows_stale = h->prec == qcb::prec::f64 && qcb::kF64FlowStoresTwoRows;
@@ -592,8 +558,8 @@ extern "C" int qcbf_get_force(qcbf_flow* h, int i, double* out) {
 // ---- observables ---------------------------------------------------------------
 static int measure_into(qcbf_flow* h, double* dout, bool clover) {
     qcbf_event* ev;
-    int rc = wait_reduction(h);
-    if (rc || (rc = timing_begin(h, QCBF_K_MEASURE, &ev))) return rc;
+    int rc = timing_begin(h, QCBF_K_MEASURE, &ev);
+    if (rc) return rc;
     HIPCHK(h, qcb::launch_measure(h->prec, h->U[h->cur], h->g, h->partials, dout, clover, h->stream));
     return timing_end(h, ev);
 }
@@ -670,7 +636,6 @@ extern "C" int qcbf_flow_measure(qcbf_flow* h, double eps, int n_steps_gf, int n
         }
     }
     std::vector<double> r((size_t)n_steps_gf * 3);
-    if ((rc = wait_reduction(h))) return rc;
     HIPCHK(h, hipMemcpyAsync(r.data(), h->dres, r.size() * sizeof(double), hipMemcpyDeviceToHost,
                              h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -697,8 +662,6 @@ extern "C" int qcbf_exp_warnings(qcbf_flow* h, int* flagged) {
 extern "C" int qcbf_unitarity(qcbf_flow* h, double* max_dev, double* max_det_dev) {
     CHECK_H(h);
     if (!max_dev || !max_det_dev) return fail(h, QCBF_ERR_ARG, "null output");
-    const int rc = wait_reduction(h);
-    if (rc) return rc;
     HIPCHK(h, qcb::launch_unitarity(h->prec, h->U[h->cur], h->g, h->partials, h->dres, h->stream));
     double r[2];
     HIPCHK(h, hipMemcpyAsync(r, h->dres, sizeof(r), hipMemcpyDeviceToHost, h->stream));
@@ -710,8 +673,6 @@ extern "C" int qcbf_unitarity(qcbf_flow* h, double* max_dev, double* max_det_dev
 
 extern "C" int qcbf_synchronize(qcbf_flow* h) {
     CHECK_H(h);
-    const int rc = wait_reduction(h);
-    if (rc) return rc;
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return QCBF_OK;
 }
