@@ -218,6 +218,42 @@ def test_large_lattice_properties_32x32x32x32():
         assert abs(p3 - p0) < 1e-13 and abs(e3 - e0) < 1e-12 * e0 and abs(q3 - q0) < 1e-10
 
 
+def test_largest_lattice_properties_64x64x64x64():
+    """The 64^4 configuration (16.8M sites, 23.6 GB per FP64 handle), on the
+    device only: unitarity after two RK3 steps, P, E, Q unchanged by a
+    periodic shift, the flow commuting with the shift bit for bit (every link
+    is formed by the same code wherever it sits, across the periodic
+    boundary too), and the FP32-mixed form within its gates."""
+    L = (64, 64, 64, 64)
+    s = (5, 17, 33, 63)
+    say("64^4: hot start, 2 RK3 steps, unitarity")
+    with qcbflow.GradientFlow(L, 0.02) as a, qcbflow.GradientFlow(L, 0.02) as b:
+        a.hot_start(2026)
+        b.copy_from(a)
+        b.shift(s)
+        a.rk3_step(2)
+        du, dd = a.unitarity()
+        assert du < 1e-13 and dd < 1e-13
+        p0, e0, q0 = a.measure()
+        say("64^4: flow of the shifted field")
+        b.rk3_step(2)
+        p1, e1, q1 = b.measure()
+        assert abs(p1 - p0) < 1e-13 and abs(e1 - e0) < 1e-12 * e0 and abs(q1 - q0) < 1e-10
+        say("64^4: shift of the flowed field, compared link by link")
+        a.shift(s)
+        assert np.array_equal(a.field(), b.field())
+    say("64^4: FP32-mixed against FP64, 2 RK3 steps")
+    with qcbflow.GradientFlow(L, 0.02) as a, qcbflow.GradientFlow(L, 0.02, qcbflow.FP32_MIXED) as m:
+        a.hot_start(2026)
+        m.copy_from(a)
+        a.rk3_step(2)
+        m.rk3_step(2)
+        pa, ea, qa = a.measure()
+        pm, em, qm = m.measure()
+        say(f"  dE/E {abs(em - ea) / ea:.2e}, dQ {abs(qm - qa):.2e}, dP/P {abs(pm - pa) / pa:.2e}")
+        assert abs(em - ea) < 1e-6 * ea and abs(qm - qa) < 1e-4
+
+
 def _su3_defects(f):
     """max |U U^dag - 1| and max |det U - 1| over all links of a host field."""
     U = fields.to_cplx(f).reshape(-1, 3, 3)
