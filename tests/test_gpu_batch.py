@@ -173,3 +173,35 @@ def test_batch_errors():
             ens.hot_start(3, 1)
         with pytest.raises(qcbflow.QcbfError):
             ens.tune(qcbflow.TUNE_BLOCKS_PER_CU, 17)
+
+
+@pytest.mark.parametrize("prec", [qcbflow.FP64, qcbflow.FP32_MIXED])
+def test_full_ensemble_configuration_equals_single_lattices(prec):
+    """The 64-configuration ensemble (16 each of 8^4, 12^4, 16^4, 24^4; 6.75M
+    sites), device hot starts and two heatbath sweeps each, flowed together
+    with graph-replayed steps and measured every step: every member's series
+    equals its single-lattice series bit for bit (the 8^4 and 16^4 members
+    take the FP64 x-row lane shuffles there, the ensemble kernels load)."""
+    say(f"64-configuration ensemble, precision {prec}")
+    dims = [((8, 12, 16, 24)[c // 16],) * 4 for c in range(64)]
+    singles = []
+    with qcbflow.Ensemble(dims, 0.02, prec) as ens:
+        for c, L in enumerate(dims):
+            src = qcbflow.GradientFlow(L, 0.02)
+            src.hot_start(100 + c)
+            src.heatbath((5.8, 6.0, 6.2)[c % 3], 2, 300 + c)
+            ens.copy_from(c, src)
+            gf = qcbflow.GradientFlow(L, 0.02, prec)
+            gf.copy_from(src)
+            src.close()
+            singles.append(gf)
+        ens.use_graph(True)
+        tqe, plaq = ens.flow_measure(4, 1)
+    try:
+        for c, gf in enumerate(singles):
+            t1, p1 = gf.flow_measure(4, 1)
+            assert np.array_equal(tqe[c], t1), c
+            assert np.array_equal(plaq[c], p1), c
+    finally:
+        for gf in singles:
+            gf.close()
