@@ -254,6 +254,30 @@ def test_largest_lattice_properties_64x64x64x64():
         assert abs(em - ea) < 1e-6 * ea and abs(qm - qa) < 1e-4
 
 
+@pytest.mark.parametrize("prec", [qcbflow.FP64, qcbflow.FP32_MIXED])
+def test_flow_commutes_with_shift_48x48x48x48(prec):
+    """The 48^4 configuration (x-extent 48: the FP64 kernels load the
+    x-neighbours), on the device: the flow of a shifted field is the shifted
+    flow, link for link, in both storage forms, and P, E, Q are unchanged."""
+    L = (48, 48, 48, 48)
+    s = (47, 1, 24, 13)
+    say(f"48^4 precision {prec}: flow and periodic shift")
+    with qcbflow.GradientFlow(L, 0.02) as src, qcbflow.GradientFlow(L, 0.02, prec) as a, \
+            qcbflow.GradientFlow(L, 0.02, prec) as b:
+        src.hot_start(11)
+        src.heatbath(6.2, 2, 11)
+        a.copy_from(src)
+        b.copy_from(src)
+        b.shift(s)
+        ta, pa = a.flow_measure(4, 1)
+        tb, pb = b.flow_measure(4, 1)
+        assert np.max(np.abs(pa - pb)) < 1e-13
+        assert np.max(np.abs(ta[:, 2] - tb[:, 2]) / ta[:, 2]) < 1e-12
+        assert np.max(np.abs(ta[:, 1] - tb[:, 1])) < 1e-10
+        a.shift(s)
+        assert np.array_equal(a.field(), b.field())
+
+
 def _su3_defects(f):
     """max |U U^dag - 1| and max |det U - 1| over all links of a host field."""
     U = fields.to_cplx(f).reshape(-1, 3, 3)
