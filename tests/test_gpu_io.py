@@ -77,6 +77,26 @@ def test_roundtrip_anisotropic_and_standard_framing(tmp_path):
     assert np.array_equal(limefile.read_ildg(q).reshape(f.shape), f)
 
 
+def test_roundtrip_across_transfer_chunks(tmp_path):
+    """80640 sites: more than one 65536-site transfer chunk, the last one
+    partial, through the double-buffered load, the chunked save, and
+    set_links / get_links."""
+    L = (20, 18, 16, 14)
+    say(f"ILDG round trip {L} across chunks")
+    f = oracle.hot_start(L, 8)
+    p = str(tmp_path / "big.ildg")
+    limefile.write_ildg(p, f, L, framing="standard")
+    with qcbflow.GradientFlow(L) as gf:
+        gf.load_ildg(p)
+        assert np.array_equal(gf.field(), f)
+        q = str(tmp_path / "big_again.ildg")
+        gf.save_ildg(q)
+        g = oracle.hot_start(L, 9)
+        gf.copy(g)
+        assert np.array_equal(gf.field(), g)
+    assert np.array_equal(limefile.read_ildg(q).reshape(f.shape), f)
+
+
 def test_load_single_precision_payload(tmp_path):
     L = (4, 4, 4, 4)
     f = oracle.hot_start(L, 6)
