@@ -95,6 +95,10 @@ def test_roundtrip_across_transfer_chunks(tmp_path):
         gf.copy(g)
         assert np.array_equal(gf.field(), g)
     assert np.array_equal(limefile.read_ildg(q).reshape(f.shape), f)
+    # the FP32-mixed form converts per chunk on the device
+    with qcbflow.GradientFlow(L, 0.02, qcbflow.FP32_MIXED) as gm:
+        gm.load_ildg(p)
+        assert np.abs(gm.field() - f).max() < 1e-6
 
 
 def test_load_single_precision_payload(tmp_path):
