@@ -137,8 +137,10 @@ def test_flow_parity_8x8x8x8_thermalised():
 
 
 # x-extents 2 and 4 divide 64: those lattices take the FP64 x-neighbour links
-# from lane shuffles, the others load them.
-@pytest.mark.parametrize("L", [(6, 4, 5, 8), (3, 5, 4, 7), (12, 4, 4, 4), (2, 4, 8, 4), (4, 6, 2, 4)])
+# from lane shuffles (when the volume fills whole 64-site tiles), the others
+# load them.  Extent 2: the neighbours x + mu and x - mu are the same site.
+@pytest.mark.parametrize("L", [(6, 4, 5, 8), (3, 5, 4, 7), (12, 4, 4, 4), (2, 4, 8, 4), (4, 6, 2, 4),
+                               (2, 2, 2, 2), (2, 2, 2, 8)])
 def test_flow_parity_anisotropic(L):
     say(f"flow parity {L}")
     f = thermal(L, 77, 5)
