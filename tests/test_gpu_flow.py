@@ -151,6 +151,26 @@ def test_flow_parity_anisotropic(L):
     check_traj(tqe, plaq, ref_tqe, ref_plaq)
 
 
+@pytest.mark.parametrize("n_gf,n_rk", [(1, 1), (4, 0), (5, 3), (0, 1)])
+def test_flow_measure_cadences(n_gf, n_rk):
+    """topo_charge_flowed's cadence (observables_mpi.cpp:159-186) at its
+    edges: one measurement and no step, repeated measurements of an unflowed
+    field, several RK3 steps between measurements (the measurement rides on
+    the first one's stage 1), and an empty series."""
+    L = (4, 4, 6, 4)
+    say(f"cadence N_steps_gf={n_gf} N_rk_steps={n_rk}")
+    f = thermal(L, 21, 2)
+    ref_tqe, ref_plaq, ref_f = oracle.flow_measure(f, L, 0.02, n_gf, n_rk, want_field=True)
+    with qcbflow.GradientFlow(L, 0.02) as gf:
+        gf.copy(f)
+        tqe, plaq = gf.flow_measure(n_gf, n_rk)
+        out = gf.field()
+    assert tqe.shape == (n_gf, 3) and plaq.shape == (n_gf,)
+    if n_gf:
+        check_traj(tqe, plaq, ref_tqe, ref_plaq)
+    assert np.abs(out - ref_f).max() < 1e-12
+
+
 def test_flow_is_deterministic():
     L = (8, 8, 8, 8)
     say("determinism 8^4")
