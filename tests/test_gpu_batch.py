@@ -205,3 +205,27 @@ def test_full_ensemble_configuration_equals_single_lattices(prec):
     finally:
         for gf in singles:
             gf.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+@pytest.mark.parametrize("n_gf,n_rk", [(1, 1), (4, 0), (5, 3)])
+def test_batch_cadences_equal_single_lattices(graph, n_gf, n_rk):
+    """The ensemble's flow_measure at the cadence edges (one measurement, no
+    steps between measurements, three between), eager and graph-replayed,
+    against each member's single-lattice series."""
+    dims = [(4, 4, 4, 4), (3, 5, 4, 7), (8, 8, 8, 8)]
+    say(f"ensemble cadence N_steps_gf={n_gf} N_rk_steps={n_rk} graph {graph}")
+    fs = fields_for(dims, sweeps=1)
+    with qcbflow.Ensemble(dims, 0.02) as ens:
+        ens.use_graph(graph)
+        for c, f in enumerate(fs):
+            ens.copy(c, f)
+        tqe, plaq = ens.flow_measure(n_gf, n_rk)
+        outs = [ens.field(c) for c in range(len(dims))]
+    for c, (L, f) in enumerate(zip(dims, fs)):
+        with qcbflow.GradientFlow(L, 0.02) as gf:
+            gf.copy(f)
+            t1, p1 = gf.flow_measure(n_gf, n_rk)
+            assert np.array_equal(tqe[c], t1), c
+            assert np.array_equal(plaq[c], p1), c
+            assert np.array_equal(outs[c], gf.field()), c
