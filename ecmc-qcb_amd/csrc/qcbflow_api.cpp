@@ -803,19 +803,21 @@ extern "C" int qcbf_copy_bench(qcbf_flow* h, size_t bytes, int reps, double* ms)
         (void)hipFree(a);
         return fail(h, QCBF_ERR_OOM, "hipMalloc: %s", hipGetErrorString(e));
     }
-    hipEvent_t e0, e1;
-    (void)hipEventCreate(&e0);
-    (void)hipEventCreate(&e1);
-    (void)hipMemsetAsync(a, 0, n * sizeof(double2), h->stream);
-    (void)qcb::launch_copy(a, b, n, h->stream);
-    (void)hipEventRecord(e0, h->stream);
-    for (int r = 0; r < reps; r++) (void)qcb::launch_copy(a, b, n, h->stream);
-    (void)hipEventRecord(e1, h->stream);
-    e = hipEventSynchronize(e1);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     float t = 0.f;
-    (void)hipEventElapsedTime(&t, e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    // the first failing call names the error; later steps are skipped
+    e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipMemsetAsync(a, 0, n * sizeof(double2), h->stream);
+    if (e == hipSuccess) e = qcb::launch_copy(a, b, n, h->stream);  // warm
+    if (e == hipSuccess) e = hipEventRecord(e0, h->stream);
+    for (int r = 0; e == hipSuccess && r < reps; r++) e = qcb::launch_copy(a, b, n, h->stream);
+    if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    if (e != hipSuccess) (void)hipStreamSynchronize(h->stream);  // nothing in flight on a, b
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
     (void)hipFree(a);
     (void)hipFree(b);
     if (e != hipSuccess) return fail(h, QCBF_ERR_HIP, "copy bench: %s", hipGetErrorString(e));
