@@ -548,6 +548,8 @@ extern "C" int qcbf_batch_tune(qcbf_batch* b, int key, long value) {
                 return bfail(b, QCBF_ERR_ARG, "tile shape %ld out of range", value);
             const int lg[4] = {(int)(value & 7), (int)((value >> 3) & 7), (int)((value >> 6) & 7),
                                (int)((value >> 9) & 7)};
+            // kernels still queued on the handle's stream read the descriptors
+            BCHK(b, hipStreamSynchronize(b->stream));
             for (auto& cd : b->cfgs) qcb::set_tile_shape(cd.g, value == -1 ? nullptr : lg);
             b->tile_shape = value;
             BCHK(b, hipMemcpy(b->d_cfgs, b->cfgs.data(), sizeof(cfg_desc) * b->n,

@@ -229,3 +229,25 @@ def test_batch_cadences_equal_single_lattices(graph, n_gf, n_rk):
             assert np.array_equal(tqe[c], t1), c
             assert np.array_equal(plaq[c], p1), c
             assert np.array_equal(outs[c], gf.field()), c
+
+
+def test_tile_shape_change_waits_for_queued_steps():
+    """qcbf_batch_tune(TILE_SHAPE) right after asynchronous RK3 steps: the
+    configuration descriptors are rewritten only after the queued kernels have
+    read them, so the result equals the synchronised sequence bit for bit."""
+    dims = [(8, 8, 8, 8), (4, 6, 8, 10), (16, 8, 8, 8)]
+    fs = fields_for(dims, sweeps=1)
+    outs = []
+    for sync in (True, False):
+        say(f"tile shape change after queued steps, synchronised first: {sync}")
+        with qcbflow.Ensemble(dims, 0.02) as ens:
+            for c, f in enumerate(fs):
+                ens.copy(c, f)
+            ens.rk3_step(6)
+            if sync:
+                ens.synchronize()
+            ens.tune(qcbflow.TUNE_TILE_SHAPE, qcbflow.tile_code(4, 2, 4, 2))
+            ens.rk3_step(2)
+            outs.append([ens.field(c) for c in range(len(dims))])
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
