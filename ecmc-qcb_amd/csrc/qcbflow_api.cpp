@@ -42,6 +42,7 @@ struct qcbf_flow {
     double* dres = nullptr;           // device results of measurements
     size_t dres_cap = 0;              // in doubles
     int* dwarn = nullptr;             // exp t > 1 flag
+    int* dstuck = nullptr;            // heatbath retry-bound flag (kept apart from dwarn)
     unsigned hb_sweeps = 0;           // heatbath sweeps done (keys the random streams)
     void* stage = nullptr;            // device staging chunk for host <-> HBM transfers
     std::string err;
@@ -135,6 +136,7 @@ static void release(qcbf_flow* h) {
     (void)hipFree(h->partials);
     (void)hipFree(h->dres);
     (void)hipFree(h->dwarn);
+    (void)hipFree(h->dstuck);
     (void)hipFree(h->stage);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -193,6 +195,7 @@ extern "C" int qcbf_create(const int dims[4], int precision, qcbf_flow** out) {
     CREATE_CHK(hipMalloc(&h->partials, (size_t)qcb::measure_blocks(h->g) * 3 * sizeof(double)));
     CREATE_CHK(hipMalloc(&h->dwarn, sizeof(int)));
     CREATE_CHK(hipMemsetAsync(h->dwarn, 0, sizeof(int), h->stream));
+    CREATE_CHK(hipMalloc(&h->dstuck, sizeof(int)));
     CREATE_CHK(hipMalloc(&h->dres, 64 * sizeof(double)));
     h->dres_cap = 64;
     if (qcb::launch_cold_start(h->prec, h->U[0], h->g, h->stream) != hipSuccess) {
@@ -330,14 +333,13 @@ extern "C" int qcbf_heatbath(qcbf_flow* h, double beta, int n_sweeps, uint64_t s
             return fail(h, QCBF_ERR_ARG, "the checkerboard heatbath needs even extents (L[%d] = %d)",
                         d, h->g.L[d]);
     if (n_sweeps < 0 || !(beta > 0.0)) return fail(h, QCBF_ERR_ARG, "bad heatbath arguments");
-    HIPCHK(h, hipMemsetAsync(h->dwarn, 0, sizeof(int), h->stream));
+    HIPCHK(h, hipMemsetAsync(h->dstuck, 0, sizeof(int), h->stream));
     HIPCHK(h, qcb::launch_heatbath_f64(static_cast<double2*>(h->U[h->cur]), h->g, beta, n_sweeps,
-                                       seed, h->hb_sweeps, h->dwarn, h->stream));
+                                       seed, h->hb_sweeps, h->dstuck, h->stream));
     h->hb_sweeps += (unsigned)n_sweeps;
     if (n_sweeps > 0) h->rows_stale = false;  // every link rewritten whole
     int stuck = 0;
-    HIPCHK(h, hipMemcpyAsync(&stuck, h->dwarn, sizeof(int), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipMemsetAsync(h->dwarn, 0, sizeof(int), h->stream));
+    HIPCHK(h, hipMemcpyAsync(&stuck, h->dstuck, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     if (stuck) return fail(h, QCBF_ERR_STATE, "an SU(2) heatbath draw hit its retry bound");
     return QCBF_OK;

@@ -204,6 +204,12 @@ def test_exp_warning_flag_on_flow():
         gf.rk3_step(1)
         assert gf.exp_warnings() == 1
         assert gf.exp_warnings() == 0  # reading resets
+        # a warning not yet read survives other calls on the handle
+        gf.hot_start(1)
+        gf.rk3_step(1)
+        gf.heatbath(6.0, 1, 3)
+        gf.measure()
+        assert gf.exp_warnings() == 1
 
 
 def test_large_lattice_properties_32x32x32x32():
