@@ -858,6 +858,16 @@ extern "C" int qcbf_autotune(qcbf_flow* h, double eps, int trial_steps, long* mo
     const int cur0 = h->cur;
     const bool timing0 = h->timing, stale0 = h->rows_stale;
     h->timing = false;
+    // the trial steps may raise the exp warning: keep the caller's flag
+    int warn0 = 0;
+    e = hipMemcpyAsync(&warn0, h->dwarn, sizeof(int), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(saveU);
+        (void)hipFree(saveA);
+        h->timing = timing0;
+        return fail(h, QCBF_ERR_HIP, "autotune: %s", hipGetErrorString(e));
+    }
     struct cand {
         int mode;
         unsigned win;
@@ -919,6 +929,8 @@ extern "C" int qcbf_autotune(qcbf_flow* h, double eps, int trial_steps, long* mo
     h->rows_stale = stale0;
     if ((e = hipMemcpyAsync(h->U[cur0], saveU, ub, hipMemcpyDeviceToDevice, h->stream)) != hipSuccess ||
         (e = hipMemcpyAsync(h->acc, saveA, ab, hipMemcpyDeviceToDevice, h->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(h->dwarn, &warn0, sizeof(int), hipMemcpyHostToDevice, h->stream)) !=
+            hipSuccess ||
         (e = hipStreamSynchronize(h->stream)) != hipSuccess)
         if (rc == QCBF_OK) rc = fail(h, QCBF_ERR_HIP, "autotune restore: %s", hipGetErrorString(e));
     (void)hipFree(saveU);

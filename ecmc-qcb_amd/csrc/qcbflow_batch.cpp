@@ -612,6 +612,15 @@ extern "C" int qcbf_batch_autotune(qcbf_batch* b, double eps, int trial_steps, l
     }
     const int cur0 = b->cur;
     const std::vector<char> stale0 = b->rows_stale;
+    // the trial steps may raise the exp warning: keep the caller's flag
+    int warn0 = 0;
+    e = hipMemcpyAsync(&warn0, b->dwarn, sizeof(int), hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(saveU);
+        (void)hipFree(saveA);
+        return bfail(b, QCBF_ERR_HIP, "autotune: %s", hipGetErrorString(e));
+    }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = QCBF_OK;
     struct cand {
@@ -682,6 +691,8 @@ extern "C" int qcbf_batch_autotune(qcbf_batch* b, double eps, int trial_steps, l
     b->rows_stale = stale0;
     if ((e = hipMemcpyAsync(b->U[cur0], saveU, ub, hipMemcpyDeviceToDevice, b->stream)) != hipSuccess ||
         (e = hipMemcpyAsync(b->acc, saveA, ab, hipMemcpyDeviceToDevice, b->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(b->dwarn, &warn0, sizeof(int), hipMemcpyHostToDevice, b->stream)) !=
+            hipSuccess ||
         (e = hipStreamSynchronize(b->stream)) != hipSuccess)
         if (rc == QCBF_OK) rc = bfail(b, QCBF_ERR_HIP, "autotune restore: %s", hipGetErrorString(e));
     (void)hipFree(saveU);

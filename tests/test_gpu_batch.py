@@ -100,6 +100,23 @@ def test_autotune_leaves_the_ensemble_unchanged():
     assert np.array_equal(t0, t1)
 
 
+def test_autotune_keeps_the_exp_warning_state():
+    """The autotuners' trial steps may raise the exp warning; the caller's
+    flag is what it was before tuning, set or clear."""
+    L = (8, 8, 8, 8)
+    with qcbflow.GradientFlow(L, 0.5) as gf, qcbflow.Ensemble([L], 0.5) as ens:
+        gf.hot_start(1)
+        ens.copy_from(0, gf)
+        gf.autotune(trial_steps=1)       # eps = 0.5 on a hot start warns in the trials
+        ens.autotune(trial_steps=1)
+        assert gf.exp_warnings() == 0 and ens.exp_warnings() == 0
+        gf.rk3_step(1)
+        ens.rk3_step(1)
+        gf.autotune(trial_steps=1)
+        ens.autotune(trial_steps=1)
+        assert gf.exp_warnings() == 1 and ens.exp_warnings() == 1
+
+
 def test_single_lattice_autotune_and_tile_maps_agree():
     L = (16, 8, 8, 12)
     with qcbflow.GradientFlow(L) as gf:
