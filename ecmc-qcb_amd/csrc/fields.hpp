@@ -119,8 +119,17 @@ __device__ inline void store_link32c(float4* __restrict__ p, unsigned V, int mu,
 __device__ inline alg8<float> load_alg32(const float4* __restrict__ p, unsigned V, int mu,
                                          unsigned s) {
     const float4* b = p + (size_t)mu * 2 * V + s;
+#if QCB_NT_ACC_LOADS
+    const float* q0 = &b[0].x;
+    const float* q1 = &b[V].x;
+    return alg8<float>{{__builtin_nontemporal_load(q0), __builtin_nontemporal_load(q0 + 1),
+                        __builtin_nontemporal_load(q0 + 2), __builtin_nontemporal_load(q0 + 3),
+                        __builtin_nontemporal_load(q1), __builtin_nontemporal_load(q1 + 1),
+                        __builtin_nontemporal_load(q1 + 2), __builtin_nontemporal_load(q1 + 3)}};
+#else
     const float4 a = b[0], c = b[V];
     return alg8<float>{{a.x, a.y, a.z, a.w, c.x, c.y, c.z, c.w}};
+#endif
 }
 __device__ inline void store_alg32(float4* __restrict__ p, unsigned V, int mu, unsigned s,
                                    const alg8<float>& a) {

@@ -315,15 +315,26 @@ __device__ inline void store_flow_link64(double2* __restrict__ p, unsigned V, in
         put2(b + (size_t)k * V, make_double2(m.e[k / 3][k % 3].r, m.e[k / 3][k % 3].i));
 }
 
+// The RK3 accumulator is read once per stage and never by a neighbour:
+// non-temporal loads keep it out of L2, which holds the neighbour links the
+// stage re-reads (measured 1.5-2% on the FP64 stage kernels at 32^4 and 64^4).
+#ifndef QCB_NT_ACC_LOADS
+#define QCB_NT_ACC_LOADS 1
+#endif
 __device__ inline alg8<double> load_alg64(const double2* __restrict__ p, unsigned V, int mu,
                                           unsigned s) {
     alg8<double> a;
     const double2* b = p + (size_t)mu * 4 * V + s;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
+#if QCB_NT_ACC_LOADS
+        a.a[2 * k] = __builtin_nontemporal_load(&b[(size_t)k * V].x);
+        a.a[2 * k + 1] = __builtin_nontemporal_load(&b[(size_t)k * V].y);
+#else
         double2 v = b[(size_t)k * V];
         a.a[2 * k] = v.x;
         a.a[2 * k + 1] = v.y;
+#endif
     }
     return a;
 }
