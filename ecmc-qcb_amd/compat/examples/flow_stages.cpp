@@ -4,8 +4,9 @@
 // replace_force_0, compute_force(1), compute_w3.  It reads an ILDG
 // configuration, runs that sequence on one GradientFlow and rk3_step on
 // another, prints the largest difference of the two flowed fields, checks
-// that rk3_step left periodic halos, and writes the host mirrors (force_0 of
-// the input, field_c after the step) as raw interior doubles for comparison.
+// that rk3_step left periodic halos, writes the host mirrors (force_0 of the
+// input, field_c after the step) as raw interior doubles for comparison, and
+// saves the flowed field with save_ildg_clime as outdir/flowed/flowed.
 //
 // usage: flow_stages <dirpath> <filename> <L> <outdir>
 // (the configuration is read from dirpath/filename/filename)
@@ -82,5 +83,6 @@ int main(int argc, char* argv[]) {
         std::fprintf(stderr, "cannot write to %s\n", outdir.c_str());
         return 1;
     }
+    save_ildg_clime("flowed", outdir, b.field_c, geo, topo);
     return bad == 0 ? 0 : 1;
 }

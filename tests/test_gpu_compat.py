@@ -76,3 +76,9 @@ def test_compat_stage_members_match_rk3_step_and_oracle(golden_dir, tmp_path):
     assert np.max(np.abs(rk3 - oracle.rk3(f.copy(), L, 0.02, 1))) < 1e-13
     stepwise = np.fromfile(tmp_path / "stepwise.bin")
     assert np.max(np.abs(stepwise - oracle.rk3(f.copy(), L, 0.02, 1))) < 1e-13
+    # save_ildg_clime of the flowed field_c: the reference's framing, the field exactly
+    saved = (tmp_path / "flowed" / "flowed").read_bytes()
+    recs = limefile.records(saved)
+    assert [(r[0], r[1], r[2]) for r in recs] == [("ildg-format", True, True),
+                                                 ("ildg-binary-data", False, True)]
+    assert np.array_equal(limefile.read_ildg(str(tmp_path / "flowed" / "flowed")), rk3)
