@@ -6,7 +6,8 @@
 // another, prints the largest difference of the two flowed fields, checks
 // that rk3_step left periodic halos, writes the host mirrors (force_0 of the
 // input, field_c after the step) as raw interior doubles for comparison, and
-// saves the flowed field with save_ildg_clime as outdir/flowed/flowed.
+// saves the flowed field with save_ildg_clime as outdir/flowed/flowed.  It
+// also prints the input's local and global plaquette, Q and E.
 //
 // usage: flow_stages <dirpath> <filename> <L> <outdir>
 // (the configuration is read from dirpath/filename/filename)
@@ -20,6 +21,7 @@
 #include "../gauge/GaugeField.h"
 #include "../io/ildg.h"
 #include "../mpi/MpiTopology.h"
+#include "../observables/observables_mpi.h"
 
 static bool write_raw(const std::string& path, const std::vector<double>& v) {
     FILE* fp = std::fopen(path.c_str(), "wb");
@@ -39,6 +41,12 @@ int main(int argc, char* argv[]) {
     GeometryCB geo(L);
     GaugeField field(geo);
     read_ildg_clime(filename, filedir, field, geo, topo);
+    // the lattice-wide observables of the input, local and global forms
+    const auto qe_local = mpi::observables::topo_q_e_clover(field, geo);
+    const auto qe_global = mpi::observables::topo_q_e_clover_global(field, geo, topo);
+    std::printf("P_local %.17g\nQE_local %.17g %.17g\nQE_global %.17g %.17g\n",
+                mpi::observables::mean_plaquette_local(field, geo), qe_local.first,
+                qe_local.second, qe_global.first, qe_global.second);
 
     // stage by stage, host mirrors refreshed after every call
     GradientFlow a(0.02, field, geo);

@@ -70,6 +70,14 @@ def test_compat_stage_members_match_rk3_step_and_oracle(golden_dir, tmp_path):
     assert "differing from their periodic image: 0" in out.stdout
     L = (4, 4, 4, 4)
     f = np.ascontiguousarray(limefile.read_ildg(conf))
+    # the observables' local and global forms (observables_mpi.cpp:11-48, 115-156)
+    vals = {l.split()[0]: [float(v) for v in l.split()[1:]] for l in out.stdout.splitlines()
+            if l.split() and l.split()[0] in ("P_local", "QE_local", "QE_global")}
+    V = 4 ** 4
+    P, (Q, E) = oracle.plaquette(f, L), oracle.q_e_clover(f, L)
+    assert abs(vals["P_local"][0] - 6 * V * P) < 1e-12 * 6 * V * abs(P)
+    assert abs(vals["QE_global"][0] - Q) < 1e-10 and abs(vals["QE_global"][1] - E) < 1e-12 * E
+    assert abs(vals["QE_local"][0] - Q) < 1e-10 and abs(vals["QE_local"][1] - V * E) < 1e-12 * V * E
     force0 = np.fromfile(tmp_path / "force0.bin")
     assert np.max(np.abs(force0 - oracle.compute_force(f.copy(), L))) < 1e-13
     rk3 = np.fromfile(tmp_path / "rk3.bin")
