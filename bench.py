@@ -89,8 +89,20 @@ def cpu_baseline(links, L, eps, steps, threads):
         "kind": "port",
         "sample": f"{steps} RK3 steps + {steps} measurements of the same "
                   f"{'x'.join(map(str, L))} FP64 field, {secs:.2f} s, "
-                  f"OpenMP {threads} threads (C oracle, gcc -O2 -fopenmp)",
+                  f"OpenMP {threads} threads of {os.cpu_count()} ({cpu_model()}), "
+                  "C oracle, gcc -O2 -fopenmp",
     }
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
 
 
 def timed_flow(h, src, K, warmup, every=1):
