@@ -198,7 +198,8 @@ def setup_shape(qcbflow, h, args):
     explicit knobs, then the autotuner if asked for"""
     for key, val in ((qcbflow.TUNE_MAP_MODE, args.map_mode),
                      (qcbflow.TUNE_MAP_WIN, args.map_win),
-                     (qcbflow.TUNE_TILE_SHAPE, args.tile_shape)):
+                     (qcbflow.TUNE_TILE_SHAPE, args.tile_shape),
+                     (qcbflow.TUNE_THREADS_PER_SITE, args.threads_per_site)):
         if val is not None:
             h.tune(key, val)
     if args.autotune:
@@ -206,7 +207,8 @@ def setup_shape(qcbflow, h, args):
     if args.no_fuse:
         h.tune(qcbflow.TUNE_FUSED_MEASURE, 0)
     return [h.get_tune(k) for k in (qcbflow.TUNE_MAP_MODE, qcbflow.TUNE_MAP_WIN,
-                                    qcbflow.TUNE_TILE_SHAPE, qcbflow.TUNE_FUSED_MEASURE)]
+                                    qcbflow.TUNE_TILE_SHAPE, qcbflow.TUNE_FUSED_MEASURE,
+                                    qcbflow.TUNE_THREADS_PER_SITE)]
 
 
 def lattice_of(args, default):
@@ -363,8 +365,12 @@ def report_ensemble(qcbflow, args):
         if args.no_fuse:
             e.tune(qcbflow.TUNE_FUSED_MEASURE, 0)
         m, w, b = (None, None, None)
+        if args.threads_per_site is not None:
+            e.tune(qcbflow.TUNE_THREADS_PER_SITE, args.threads_per_site)
         if not args.no_autotune:  # the ensemble configuration is autotuned
             m, w, b = e.autotune(trial_steps=2)
+        tps = e.get_tune(qcbflow.TUNE_THREADS_PER_SITE)
+        shape_code = e.get_tune(qcbflow.TUNE_TILE_SHAPE)
         e.synchronize()
         t0 = time.perf_counter()
         tqe, plaq = e.flow_measure(K + 1, 1)
@@ -378,7 +384,8 @@ def report_ensemble(qcbflow, args):
             "seconds": secs,
             "sequential_value": total_v * K / seq_secs[p],
             "speedup_vs_sequential": seq_secs[p] / secs,
-            "launch_shape": {"map_mode": m, "map_win": w, "blocks_per_cu": b},
+            "launch_shape": {"map_mode": m, "map_win": w, "blocks_per_cu": b,
+                             "tile_shape": shape_code, "threads_per_site": tps},
             "bit_identical_to_single_lattice_path": bool(same),
             "exp_warnings": e.exp_warnings(),
         }
@@ -419,6 +426,8 @@ def main():
                     help="workgroup->tile order (qcbf_tune); default: the library's")
     ap.add_argument("--map-win", type=int, default=None)
     ap.add_argument("--tile-shape", type=int, default=None, help="qcbf_tune tile-shape code")
+    ap.add_argument("--threads-per-site", type=int, choices=(1, 2, 4), default=None,
+                    help="threads per site of the RK3 stage kernels (default 4)")
     ap.add_argument("--autotune", action="store_true",
                     help="single lattice: run qcbf_autotune before the warmup (default: the "
                          "library's launch shape, which the autotuner matches within noise)")
@@ -469,7 +478,7 @@ def main():
     gf = qcbflow.GradientFlow(L, args.eps)
     gf.copy_from(src)
     shape64 = setup_shape(qcbflow, gf, args)
-    log(f"fp64 launch shape (map mode, window, tile code): {shape64}")
+    log(f"fp64 launch shape (map mode, window, tile code, fused, threads/site): {shape64}")
 
     # timed region: K RK3 steps, K+1 measurements, stage kernels bracketed by events
     secs, tqe, plaq, ms, launches = timed_flow(gf, src, K, args.warmup)

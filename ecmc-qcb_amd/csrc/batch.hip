@@ -34,7 +34,7 @@ namespace qcb {
         }                                                                    \
     } while (0)
 
-template <prec P, int STAGE>
+template <prec P, int STAGE, int DPW>
 __device__ __forceinline__ void batch_stage_slot(unsigned v,
                                                  const typename storage<P>::vec* __restrict__ Uin,
                                                  typename storage<P>::vec* __restrict__ Uout,
@@ -43,22 +43,17 @@ __device__ __forceinline__ void batch_stage_slot(unsigned v,
                                                  const unsigned* __restrict__ tile_cfg,
                                                  unsigned ntiles, int map_mode, unsigned map_win,
                                                  double cz, double ca, bool& w) {
-    const unsigned tile = tile_of_block(v, ntiles, map_mode, map_win);
+    // slot v is a group of DPW tiles (group_tiles); 4/DPW wavefronts per tile
+    const tile_groups G = group_tiles(ntiles, 0u, map_mode, map_win, DPW);
+    const unsigned tile = group_tile<DPW>(tile_of_block(v, G.n, map_mode, G.win), threadIdx.y);
     if (tile >= ntiles) return;
     const cfg_desc& c = cfgs[tile_cfg[tile]];
     unsigned co[4];
     const unsigned s = tile_site(c.g, tile - c.tile0, threadIdx.x, co);
     if (s >= c.g.V) return;
     const nbr n = neighbours_at(c.g, co);
-    const auto f = storage<P>::view(Uin + c.u_off, c.g.V);
-    auto* out = Uout + c.u_off;
-    auto* a = acc + c.a_off;
-    switch (threadIdx.y) {
-        case 0: stage_body<P, STAGE, 0>(f, out, a, s, n, cz, ca, w); break;
-        case 1: stage_body<P, STAGE, 1>(f, out, a, s, n, cz, ca, w); break;
-        case 2: stage_body<P, STAGE, 2>(f, out, a, s, n, cz, ca, w); break;
-        default: stage_body<P, STAGE, 3>(f, out, a, s, n, cz, ca, w); break;
-    }
+    stage_directions<P, STAGE, false, DPW>(threadIdx.y, Uin + c.u_off, Uout + c.u_off,
+                                           acc + c.a_off, c.g.V, s, n, cz, ca, w);
 }
 
 // Wavefronts per SIMD of the one-workgroup-per-tile stage kernel: the
@@ -67,7 +62,7 @@ __device__ __forceinline__ void batch_stage_slot(unsigned v,
 template <prec P>
 constexpr int kFlatStageWaves = P == prec::f64 ? 3 : 6;
 
-template <prec P, int STAGE, bool PERSIST>
+template <prec P, int STAGE, bool PERSIST, int DPW>
 __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(PERSIST ? 1 : kFlatStageWaves<P>))) void k_batch_stage(const typename storage<P>::vec* __restrict__ Uin,
                                                      typename storage<P>::vec* __restrict__ Uout,
@@ -77,26 +72,44 @@ __attribute__((amdgpu_waves_per_eu(PERSIST ? 1 : kFlatStageWaves<P>))) void k_ba
                                                      unsigned ntiles, int map_mode, unsigned map_win,
                                                      double cz, double ca, int* __restrict__ warn) {
     bool w = false;
-    const unsigned nslots = grid_for_tiles(ntiles, map_mode, map_win);
+    const tile_groups G = group_tiles(ntiles, 0u, map_mode, map_win, DPW);
+    const unsigned nslots = grid_for_tiles(G.n, map_mode, G.win);
     QCB_FOR_EACH_SLOT(PERSIST, nslots,
-                      (batch_stage_slot<P, STAGE>(v, Uin, Uout, acc, cfgs, tile_cfg, ntiles,
-                                                  map_mode, map_win, cz, ca, w)));
+                      (batch_stage_slot<P, STAGE, DPW>(v, Uin, Uout, acc, cfgs, tile_cfg, ntiles,
+                                                       map_mode, map_win, cz, ca, w)));
     if (w) atomicOr(warn, 1);
 }
 
-template <prec P, int STAGE>
-static void batch_stage_kernel(const batch_launch& bl, unsigned nslots, hipStream_t st,
+template <prec P, int STAGE, int DPW>
+static void batch_stage_kernel(const batch_launch& bl, hipStream_t st,
                                const typename storage<P>::vec* in, typename storage<P>::vec* out,
                                typename storage<P>::avec* a, const cfg_desc* cfgs,
                                const unsigned* tile_cfg, unsigned ntiles, double cz, double ca,
                                int* warn) {
     const dim3 block(kSitesPerBlock, 4);
+    const tile_groups G = group_tiles(ntiles, 0u, bl.map_mode, bl.map_win, DPW);
+    const unsigned nslots = grid_for_tiles(G.n, bl.map_mode, G.win);
     if (bl.grid)
-        k_batch_stage<P, STAGE, true><<<bl.grid, block, 0, st>>>(
+        k_batch_stage<P, STAGE, true, DPW><<<bl.grid, block, 0, st>>>(
             in, out, a, cfgs, tile_cfg, ntiles, bl.map_mode, bl.map_win, cz, ca, warn);
     else
-        k_batch_stage<P, STAGE, false><<<nslots, block, 0, st>>>(
+        k_batch_stage<P, STAGE, false, DPW><<<nslots, block, 0, st>>>(
             in, out, a, cfgs, tile_cfg, ntiles, bl.map_mode, bl.map_win, cz, ca, warn);
+}
+
+template <prec P, int STAGE>
+static hipError_t batch_stage_kernel(const batch_launch& bl, hipStream_t st,
+                                     const typename storage<P>::vec* in,
+                                     typename storage<P>::vec* out, typename storage<P>::avec* a,
+                                     const cfg_desc* cfgs, const unsigned* tile_cfg,
+                                     unsigned ntiles, double cz, double ca, int* warn) {
+    switch (bl.dpw) {
+        case 1: batch_stage_kernel<P, STAGE, 1>(bl, st, in, out, a, cfgs, tile_cfg, ntiles, cz, ca, warn); break;
+        case 2: batch_stage_kernel<P, STAGE, 2>(bl, st, in, out, a, cfgs, tile_cfg, ntiles, cz, ca, warn); break;
+        case 4: batch_stage_kernel<P, STAGE, 4>(bl, st, in, out, a, cfgs, tile_cfg, ntiles, cz, ca, warn); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipSuccess;
 }
 
 template <prec P>
@@ -108,14 +121,14 @@ static hipError_t batch_stage_launch(int stage, const void* Uin, void* Uout, voi
     const auto* in = static_cast<const typename S::vec*>(Uin);
     auto* out = static_cast<typename S::vec*>(Uout);
     auto* a = static_cast<typename S::avec*>(acc);
-    const unsigned nslots = grid_for_tiles(ntiles, bl.map_mode, bl.map_win);
+    hipError_t e;
     switch (stage) {
-        case 1: batch_stage_kernel<P, 1>(bl, nslots, st, in, out, a, cfgs, tile_cfg, ntiles, cz, ca, warn); break;
-        case 2: batch_stage_kernel<P, 2>(bl, nslots, st, in, out, a, cfgs, tile_cfg, ntiles, cz, ca, warn); break;
-        case 3: batch_stage_kernel<P, 3>(bl, nslots, st, in, out, a, cfgs, tile_cfg, ntiles, cz, ca, warn); break;
+        case 1: e = batch_stage_kernel<P, 1>(bl, st, in, out, a, cfgs, tile_cfg, ntiles, cz, ca, warn); break;
+        case 2: e = batch_stage_kernel<P, 2>(bl, st, in, out, a, cfgs, tile_cfg, ntiles, cz, ca, warn); break;
+        case 3: e = batch_stage_kernel<P, 3>(bl, st, in, out, a, cfgs, tile_cfg, ntiles, cz, ca, warn); break;
         default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
+    return e != hipSuccess ? e : hipGetLastError();
 }
 
 hipError_t launch_batch_stage(prec p, int stage, const void* Uin, void* Uout, void* acc,

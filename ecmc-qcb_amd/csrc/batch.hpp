@@ -26,6 +26,7 @@ struct batch_launch {
     unsigned grid;      // workgroups of the persistent grid; 0: one workgroup per tile
     int map_mode;       // tile order (tile_of_block)
     unsigned map_win;
+    int dpw;            // directions per wavefront of the stage kernels (1, 2, 4)
 };
 
 hipError_t launch_batch_stage(prec p, int stage, const void* Uin, void* Uout, void* acc,

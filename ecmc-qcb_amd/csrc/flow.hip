@@ -58,32 +58,27 @@ inline bool use_x_rows(const lattice& g) {
     return QCB_XROW_SHUFFLES && P == prec::f64 && x_rows_in_wave(g);
 }
 
-template <prec P, int STAGE, bool XROWS>
+template <prec P, int STAGE, bool XROWS, int DPW>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kStageWaves<P>))) void k_rk3_stage(
     const typename storage<P>::vec* __restrict__ Uin, typename storage<P>::vec* __restrict__ Uout,
     typename storage<P>::avec* __restrict__ acc, lattice g, double cz, double ca,
     int* __restrict__ warn) {
     const unsigned ntiles = stage_tiles(g);
-    const unsigned tile =
-        tile_of_block(blockIdx.x, ntiles, g.map_mode, g.map_win, stage_tiles_per_t(g));
-    if (tile >= ntiles) return;
+    const tile_groups G = stage_groups(g);
+    const unsigned grp = tile_of_block(blockIdx.x, G.n, g.map_mode, G.win, G.per_t);
+    const unsigned tile = group_tile<DPW>(grp, threadIdx.y);
+    if (tile >= ntiles) return;  // padding group, or past the last tile
     unsigned c[4];
     const unsigned s = tile_site(g, tile, threadIdx.x, c);
     if (s >= g.V) return;  // never a partial wavefront when XROWS (full tiles)
     nbr n = neighbours_at(g, c);
     if constexpr (XROWS) set_x_lanes(n, g, threadIdx.x);
-    const auto f = storage<P>::view(Uin, g.V);
     bool w = false;
-    switch (threadIdx.y) {
-        case 0: stage_body<P, STAGE, 0, XROWS>(f, Uout, acc, s, n, cz, ca, w); break;
-        case 1: stage_body<P, STAGE, 1, XROWS>(f, Uout, acc, s, n, cz, ca, w); break;
-        case 2: stage_body<P, STAGE, 2, XROWS>(f, Uout, acc, s, n, cz, ca, w); break;
-        default: stage_body<P, STAGE, 3, XROWS>(f, Uout, acc, s, n, cz, ca, w); break;
-    }
+    stage_directions<P, STAGE, XROWS, DPW>(threadIdx.y, Uin, Uout, acc, g.V, s, n, cz, ca, w);
     if (w) atomicOr(warn, 1);
 }
 
-template <prec P>
+template <prec P, int DPW>
 static hipError_t stage_launch(int stage, const void* Uin, void* Uout, void* acc, const lattice& g,
                                double cz, double ca, int* warn, hipStream_t st) {
     using S = storage<P>;
@@ -91,19 +86,31 @@ static hipError_t stage_launch(int stage, const void* Uin, void* Uout, void* acc
     auto* out = static_cast<typename S::vec*>(Uout);
     auto* a = static_cast<typename S::avec*>(acc);
     const dim3 block(kSitesPerBlock, 4);
-    const dim3 grid(grid_for_tiles(stage_tiles(g), g.map_mode, g.map_win, stage_tiles_per_t(g)));
+    const tile_groups G = stage_groups(g);
+    const dim3 grid(grid_for_tiles(G.n, g.map_mode, G.win, G.per_t));
     const bool xr = use_x_rows<P>(g);
     constexpr bool XR = P == prec::f64;  // only FP64 has x-row variants
     switch (stage * 2 + (xr ? 1 : 0)) {
-        case 2: k_rk3_stage<P, 1, false><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
-        case 3: k_rk3_stage<P, 1, XR><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
-        case 4: k_rk3_stage<P, 2, false><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
-        case 5: k_rk3_stage<P, 2, XR><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
-        case 6: k_rk3_stage<P, 3, false><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
-        case 7: k_rk3_stage<P, 3, XR><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
+        case 2: k_rk3_stage<P, 1, false, DPW><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
+        case 3: k_rk3_stage<P, 1, XR, DPW><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
+        case 4: k_rk3_stage<P, 2, false, DPW><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
+        case 5: k_rk3_stage<P, 2, XR, DPW><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
+        case 6: k_rk3_stage<P, 3, false, DPW><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
+        case 7: k_rk3_stage<P, 3, XR, DPW><<<grid, block, 0, st>>>(in, out, a, g, cz, ca, warn); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+template <prec P>
+static hipError_t stage_launch(int stage, const void* Uin, void* Uout, void* acc, const lattice& g,
+                               double cz, double ca, int* warn, hipStream_t st) {
+    switch (g.dpw) {
+        case 1: return stage_launch<P, 1>(stage, Uin, Uout, acc, g, cz, ca, warn, st);
+        case 2: return stage_launch<P, 2>(stage, Uin, Uout, acc, g, cz, ca, warn, st);
+        case 4: return stage_launch<P, 4>(stage, Uin, Uout, acc, g, cz, ca, warn, st);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_rk3_stage(prec p, int stage, const void* Uin, void* Uout, void* acc,

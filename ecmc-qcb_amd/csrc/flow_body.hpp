@@ -233,4 +233,61 @@ __device__ inline void stage_body(const typename storage<P>::field& f,
     S::store(Uout, f.V, MU, s, mul(e, u));
 }
 
+// The site index through an empty asm statement: the compiler must assume it
+// changed, so loads addressed from it are never merged with earlier loads.
+__device__ __forceinline__ unsigned opaque_site(unsigned s) {
+    asm volatile("" : "+v"(s));
+    return s;
+}
+
+// One of several direction bodies run back to back by one wavefront.  Merged
+// with the previous body's loads, the links the directions share would stay
+// live across bodies and spill; through opaque_site and behind a scheduling
+// barrier each body needs only the registers of a one-direction wavefront and
+// reads the shared links again (from the L1).
+template <prec P, int STAGE, int MU, bool XROWS>
+__device__ __forceinline__ void stage_body_next(const typename storage<P>::field& f,
+                                                typename storage<P>::vec* __restrict__ Uout,
+                                                typename storage<P>::avec* __restrict__ acc,
+                                                unsigned s, const nbr& n, double cz, double ca,
+                                                bool& warn) {
+    __builtin_amdgcn_sched_barrier(0);
+    stage_body<P, STAGE, MU, XROWS>(f, Uout, acc, opaque_site(s), n, cz, ca, warn);
+}
+
+// The directions of site s that wavefront y (threadIdx.y) updates: with
+// q = y mod 4/DPW, directions q*DPW .. q*DPW + DPW - 1 (threads per site =
+// 4/DPW).
+template <prec P, int STAGE, bool XROWS, int DPW>
+__device__ __forceinline__ void stage_directions(unsigned y,
+                                                 const typename storage<P>::vec* __restrict__ Uin,
+                                                 typename storage<P>::vec* __restrict__ Uout,
+                                                 typename storage<P>::avec* __restrict__ acc,
+                                                 unsigned V, unsigned s, const nbr& n, double cz,
+                                                 double ca, bool& w) {
+    static_assert(DPW == 1 || DPW == 2 || DPW == 4, "1, 2 or 4 directions per wavefront");
+    const auto f = storage<P>::view(Uin, V);
+    if constexpr (DPW == 1) {
+        switch (y) {
+            case 0: stage_body<P, STAGE, 0, XROWS>(f, Uout, acc, s, n, cz, ca, w); break;
+            case 1: stage_body<P, STAGE, 1, XROWS>(f, Uout, acc, s, n, cz, ca, w); break;
+            case 2: stage_body<P, STAGE, 2, XROWS>(f, Uout, acc, s, n, cz, ca, w); break;
+            default: stage_body<P, STAGE, 3, XROWS>(f, Uout, acc, s, n, cz, ca, w); break;
+        }
+    } else if constexpr (DPW == 2) {
+        if (__builtin_amdgcn_readfirstlane((int)y) % 2 == 0) {
+            stage_body_next<P, STAGE, 0, XROWS>(f, Uout, acc, s, n, cz, ca, w);
+            stage_body_next<P, STAGE, 1, XROWS>(f, Uout, acc, s, n, cz, ca, w);
+        } else {
+            stage_body_next<P, STAGE, 2, XROWS>(f, Uout, acc, s, n, cz, ca, w);
+            stage_body_next<P, STAGE, 3, XROWS>(f, Uout, acc, s, n, cz, ca, w);
+        }
+    } else {
+        stage_body_next<P, STAGE, 0, XROWS>(f, Uout, acc, s, n, cz, ca, w);
+        stage_body_next<P, STAGE, 1, XROWS>(f, Uout, acc, s, n, cz, ca, w);
+        stage_body_next<P, STAGE, 2, XROWS>(f, Uout, acc, s, n, cz, ca, w);
+        stage_body_next<P, STAGE, 3, XROWS>(f, Uout, acc, s, n, cz, ca, w);
+    }
+}
+
 }  // namespace qcb

@@ -32,6 +32,9 @@ struct lattice {
     // box tile needs every L[d] divisible by its extent.
     int lg_tile[4];
     unsigned ntile[4];  // tiles along each direction (box tiles)
+    // Directions per wavefront of the plain stage kernels (1, 2 or 4; threads
+    // per site = 4 / dpw): a workgroup of four wavefronts takes dpw tiles.
+    int dpw;
 };
 
 // Site and coordinates of lane `lane` (0..63) of flow-kernel tile `tile`.
@@ -139,6 +142,7 @@ inline lattice make_lattice(const int dims[4]) {
         g.lg_tile[d] = d == 0 ? -1 : 0;
         g.ntile[d] = 0;
     }
+    g.dpw = 1;
     // Slab sweep with 16 slabs per t-slice where the tiles allow it, else XCD
     // chunks in windows of 1024 (measured on MI355X at 32^4, 48^4 and 64^4;
     // qcbf_autotune refines it).
@@ -223,6 +227,40 @@ inline bool set_tile_shape(lattice& g, const int* lg) {
 }
 __host__ __device__ inline unsigned stage_tiles(const lattice& g) {
     return g.lg_tile[0] < 0 ? (g.V + 63) / 64 : g.ntile[0] * g.ntile[1] * g.ntile[2] * g.ntile[3];
+}
+
+// A plain-stage workgroup with dpw directions per wavefront covers a group of
+// dpw consecutive tiles (4/dpw wavefronts each).  The XCD-aware order then
+// maps tile groups: n groups, per_t groups per t-slice (0 when a group would
+// straddle two t-slices, which falls back to the linear order in mode 3), and
+// a mode-3 slab of win/dpw groups, the same number of sites as win tiles.
+struct tile_groups {
+    unsigned n, per_t, win;
+};
+__host__ __device__ inline tile_groups group_tiles(unsigned ntiles, unsigned per_t, int mode,
+                                                   unsigned win, unsigned dpw) {
+    tile_groups r;
+    r.n = (ntiles + dpw - 1) / dpw;
+    r.per_t = per_t % dpw == 0 ? per_t / dpw : 0u;
+    r.win = mode == 3 ? (win / dpw < 1u ? 1u : win / dpw) : win;
+    return r;
+}
+// Tile of wavefront y (the row threadIdx.y of a 64 x 4 block) in tile group
+// grp: 4/dpw wavefronts per tile.
+__host__ __device__ inline unsigned tile_in_group(unsigned grp, unsigned y, unsigned dpw) {
+    return grp * dpw + y / (4u / dpw);
+}
+// A wavefront is one row y of the block, so y is wavefront-uniform;
+// readfirstlane tells the compiler, which keeps the tile and what is read
+// through it (the ensemble descriptors) in scalar registers.
+template <int DPW>
+__device__ inline unsigned group_tile(unsigned grp, unsigned y) {
+    if constexpr (DPW == 1) return grp;
+    return tile_in_group(grp, (unsigned)__builtin_amdgcn_readfirstlane((int)y), DPW);
+}
+__host__ __device__ inline tile_groups stage_groups(const lattice& g) {
+    return group_tiles(stage_tiles(g), stage_tiles_per_t(g), g.map_mode, g.map_win,
+                       (unsigned)g.dpw);
 }
 
 // ---- FP64 structure-of-arrays link field -----------------------------------

@@ -727,6 +727,11 @@ extern "C" int qcbf_tune(qcbf_flow* h, int key, long value) {
             if (value != 0 && value != 1) return fail(h, QCBF_ERR_ARG, "fused measure %ld not 0 or 1", value);
             h->fuse_measure = value == 1;
             return QCBF_OK;
+        case QCBF_TUNE_THREADS_PER_SITE:
+            if (value != 1 && value != 2 && value != 4)
+                return fail(h, QCBF_ERR_ARG, "threads per site %ld not 1, 2 or 4", value);
+            h->g.dpw = 4 / (int)value;
+            return QCBF_OK;
         case QCBF_TUNE_TILE_SHAPE: {
             if (value == -1) {
                 qcb::set_tile_shape(h->g, nullptr);
@@ -757,6 +762,7 @@ extern "C" int qcbf_get_tune(const qcbf_flow* h, int key, long* value) {
         case QCBF_TUNE_MAP_WIN: *value = h->g.map_win; return QCBF_OK;
         case QCBF_TUNE_TILE_SHAPE: *value = tile_code(h->g); return QCBF_OK;
         case QCBF_TUNE_FUSED_MEASURE: *value = h->fuse_measure ? 1 : 0; return QCBF_OK;
+        case QCBF_TUNE_THREADS_PER_SITE: *value = 4 / h->g.dpw; return QCBF_OK;
         default: return fail(nullptr, QCBF_ERR_ARG, "unknown tuning key %d", key);
     }
 }
@@ -872,7 +878,10 @@ extern "C" int qcbf_autotune(qcbf_flow* h, double eps, int trial_steps, long* mo
         int mode;
         unsigned win;
         int lg[4];  // lg[0] < 0: 64 consecutive sites
+        int dpw;    // directions per wavefront (4 / threads per site)
     };
+    // tile order x tile shape at four threads per site, then two and one
+    // thread per site on the best of those
     std::vector<cand> cands;
     const std::pair<int, unsigned> maps[] = {{0, 64}, {1, 64}, {2, 256}, {2, 1024}, {2, 4096}};
     const int shapes[][4] = {{-1, 0, 0, 0}, {2, 2, 1, 1}, {3, 2, 1, 0},
@@ -881,14 +890,16 @@ extern "C" int qcbf_autotune(qcbf_flow* h, double eps, int trial_steps, long* mo
         qcb::lattice probe = h->g;
         if (sh[0] >= 0 && !qcb::set_tile_shape(probe, sh)) continue;
         if (sh[0] < 0) qcb::set_tile_shape(probe, nullptr);
-        for (const auto& m : maps) cands.push_back({m.first, m.second, {sh[0], sh[1], sh[2], sh[3]}});
+        for (const auto& m : maps) cands.push_back({m.first, m.second, {sh[0], sh[1], sh[2], sh[3]}, 1});
         // slab sweeps: 8, 16 or 32 slabs per t-slice (one, two or four per XCD)
         const unsigned pt = qcb::stage_tiles_per_t(probe);
         for (unsigned ns : {8u, 16u, 32u})
-            if (pt && pt % ns == 0) cands.push_back({3, pt / ns, {sh[0], sh[1], sh[2], sh[3]}});
+            if (pt && pt % ns == 0) cands.push_back({3, pt / ns, {sh[0], sh[1], sh[2], sh[3]}, 1});
     }
+    const size_t first_pass = cands.size();
     const qcb::lattice g0 = h->g;
-    cand best = {g0.map_mode, g0.map_win, {g0.lg_tile[0], g0.lg_tile[1], g0.lg_tile[2], g0.lg_tile[3]}};
+    cand best = {g0.map_mode, g0.map_win, {g0.lg_tile[0], g0.lg_tile[1], g0.lg_tile[2], g0.lg_tile[3]},
+                 g0.dpw};
     float best_ms = 1e30f;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = QCBF_OK;
@@ -896,10 +907,11 @@ extern "C" int qcbf_autotune(qcbf_flow* h, double eps, int trial_steps, long* mo
         (e = hipMemcpyAsync(saveA, h->acc, ab, hipMemcpyDeviceToDevice, h->stream)) != hipSuccess ||
         (e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess)
         rc = fail(h, QCBF_ERR_HIP, "autotune setup: %s", hipGetErrorString(e));
-    for (const auto& c : cands) {
-        if (rc) break;
+    for (size_t k = 0; rc == QCBF_OK && k < cands.size(); k++) {
+        const cand c = cands[k];
         h->g.map_mode = c.mode;
         h->g.map_win = c.win;
+        h->g.dpw = c.dpw;
         qcb::set_tile_shape(h->g, c.lg[0] < 0 ? nullptr : c.lg);
         rc = rk3_once(h, eps);  // warm
         if (rc) break;
@@ -920,9 +932,16 @@ extern "C" int qcbf_autotune(qcbf_flow* h, double eps, int trial_steps, long* mo
             best_ms = ms;
             best = c;
         }
+        if (k + 1 == first_pass)
+            for (int d : {2, 4}) {
+                cand t = best;
+                t.dpw = d;
+                cands.push_back(t);
+            }
     }
     h->g.map_mode = best.mode;
     h->g.map_win = best.win;
+    h->g.dpw = best.dpw;
     qcb::set_tile_shape(h->g, best.lg[0] < 0 ? nullptr : best.lg);
     h->cur = cur0;
     h->timing = timing0;

@@ -44,7 +44,7 @@ struct qcbf_batch {
     unsigned* d_slot = nullptr;
     int* dwarn = nullptr;
     void* stage = nullptr;
-    qcb::batch_launch bl{0, 2, 1024};
+    qcb::batch_launch bl{0, 2, 1024, 1};
     int blocks_per_cu = 0;  // one workgroup per tile: the autotuner's pick on MI355X
     long tile_shape = -1;
     int cus = 256;
@@ -543,6 +543,11 @@ extern "C" int qcbf_batch_tune(qcbf_batch* b, int key, long value) {
             if (value != 0 && value != 1) return bfail(b, QCBF_ERR_ARG, "fused measure %ld not 0 or 1", value);
             b->fuse_measure = value == 1;
             break;
+        case QCBF_TUNE_THREADS_PER_SITE:
+            if (value != 1 && value != 2 && value != 4)
+                return bfail(b, QCBF_ERR_ARG, "threads per site %ld not 1, 2 or 4", value);
+            b->bl.dpw = 4 / (int)value;
+            break;
         case QCBF_TUNE_TILE_SHAPE: {
             if (value != -1 && (value < 0 || value >= 4096))
                 return bfail(b, QCBF_ERR_ARG, "tile shape %ld out of range", value);
@@ -570,6 +575,7 @@ extern "C" int qcbf_batch_get_tune(const qcbf_batch* b, int key, long* value) {
         case QCBF_TUNE_BLOCKS_PER_CU: *value = b->blocks_per_cu; return QCBF_OK;
         case QCBF_TUNE_TILE_SHAPE: *value = b->tile_shape; return QCBF_OK;
         case QCBF_TUNE_FUSED_MEASURE: *value = b->fuse_measure ? 1 : 0; return QCBF_OK;
+        case QCBF_TUNE_THREADS_PER_SITE: *value = 4 / b->bl.dpw; return QCBF_OK;
         default: return QCBF_ERR_ARG;
     }
 }
@@ -628,13 +634,17 @@ extern "C" int qcbf_batch_autotune(qcbf_batch* b, double eps, int trial_steps, l
         unsigned win;
         int bpc;
         long shape;
+        int dpw;
     };
+    // tile order x grid x tile shape at four threads per site, then two and
+    // one thread per site on the best of those
     std::vector<cand> cands;
     const std::pair<int, unsigned> maps[] = {{0, 64}, {2, 256}, {2, 1024}};
     for (auto m : maps)
         for (int bpc : {0, 1, 2, 3, 4})
-            for (long sh : {-1L, 594L, 83L}) cands.push_back({m.first, m.second, bpc, sh});
-    cand best = {b->bl.map_mode, b->bl.map_win, b->blocks_per_cu, b->tile_shape};
+            for (long sh : {-1L, 594L, 83L}) cands.push_back({m.first, m.second, bpc, sh, 1});
+    const size_t first_pass = cands.size();
+    cand best = {b->bl.map_mode, b->bl.map_win, b->blocks_per_cu, b->tile_shape, b->bl.dpw};
     auto apply_shape = [&](long code) -> hipError_t {
         const int lg[4] = {(int)(code & 7), (int)((code >> 3) & 7), (int)((code >> 6) & 7),
                            (int)((code >> 9) & 7)};
@@ -652,6 +662,7 @@ extern "C" int qcbf_batch_autotune(qcbf_batch* b, double eps, int trial_steps, l
         b->bl.map_mode = cands[k].mode;
         b->bl.map_win = cands[k].win;
         b->blocks_per_cu = cands[k].bpc;
+        b->bl.dpw = cands[k].dpw;
         set_grid(b);
         if ((e = apply_shape(cands[k].shape)) != hipSuccess) {
             rc = bfail(b, QCBF_ERR_HIP, "autotune shape: %s", hipGetErrorString(e));
@@ -680,10 +691,17 @@ extern "C" int qcbf_batch_autotune(qcbf_batch* b, double eps, int trial_steps, l
             best_ms = ms;
             best = cands[k];
         }
+        if (k + 1 == first_pass)
+            for (int d : {2, 4}) {
+                cand c = best;
+                c.dpw = d;
+                cands.push_back(c);
+            }
     }
     b->bl.map_mode = best.mode;
     b->bl.map_win = best.win;
     b->blocks_per_cu = best.bpc;
+    b->bl.dpw = best.dpw;
     set_grid(b);
     if ((e = apply_shape(best.shape)) != hipSuccess && rc == QCBF_OK)
         rc = bfail(b, QCBF_ERR_HIP, "autotune shape: %s", hipGetErrorString(e));

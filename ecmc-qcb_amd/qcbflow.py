@@ -33,6 +33,7 @@ FP32_MIXED = 1
 K_STAGE1, K_STAGE2, K_STAGE3, K_MEASURE, K_STAGE1_MEASURE = 0, 1, 2, 3, 4
 TUNE_MAP_MODE, TUNE_MAP_WIN, TUNE_BLOCKS_PER_CU, TUNE_TILE_SHAPE = 0, 1, 2, 3
 TUNE_FUSED_MEASURE = 4
+TUNE_THREADS_PER_SITE = 5
 
 
 def tile_code(bx, by, bz, bt):
@@ -317,7 +318,9 @@ class GradientFlow:
         return v.value
 
     def autotune(self, trial_steps=3, eps=None):
-        """Pick the fastest tile order (field restored afterwards); returns (mode, win)."""
+        """Pick the fastest launch shape: tile order, tile shape, then threads per
+        site (field restored afterwards); returns (mode, win), the rest through
+        get_tune."""
         m, w = ctypes.c_long(), ctypes.c_long()
         self._c(load().qcbf_autotune(self._h, self.epsilon if eps is None else float(eps),
                                      int(trial_steps), ctypes.byref(m), ctypes.byref(w)))

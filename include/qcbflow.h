@@ -198,11 +198,16 @@ enum qcbf_tune_key {
                                     next RK3 stage 1: 1 (default) or 0 (separate
                                     kernels); results are bit-identical either way.
                                     Needs the default tile shape. */
-    QCBF_TUNE_TILE_SHAPE = 3     /* sites of one flow workgroup: -1 = 64 consecutive
+    QCBF_TUNE_TILE_SHAPE = 3,    /* sites of one flow workgroup: -1 = 64 consecutive
                                     sites, else a 4-D box of extents 2^a x 2^b x 2^c x 2^d
                                     (a+b+c+d = 6) encoded a + 8b + 64c + 512d; the box
                                     must divide the lattice (ensembles: applied to each
                                     member it divides) */
+    QCBF_TUNE_THREADS_PER_SITE = 5 /* threads per site of the RK3 stage kernels: 4
+                                      (default; one per direction), 2 or 1 (each thread
+                                      updates 2 or 4 directions in turn, and a workgroup
+                                      takes 2 or 4 tiles).  The fused measurement stage
+                                      keeps 4. */
 };
 /* Set one launch parameter of h; results do not depend on it (only the order in
  * which workgroups visit sites changes).  QCBF_ERR_ARG for an unknown key or
@@ -211,9 +216,11 @@ int qcbf_tune(qcbf_flow* h, int key, long value);
 int qcbf_get_tune(const qcbf_flow* h, int key, long* value);
 
 /* Time each candidate launch shape of the flow kernels (tile order x 4-D tile
- * shape, see qcbf_tune_key) for trial_steps RK3 steps on the real field, which
+ * shape at 4 threads per site, then 2 and 1 thread per site on the fastest of
+ * those; see qcbf_tune_key) for trial_steps RK3 steps on the real field, which
  * is saved first and restored afterwards, and keep the fastest; *mode and *win
- * (may be NULL) receive the chosen order, qcbf_get_tune the tile shape. */
+ * (may be NULL) receive the chosen order, qcbf_get_tune the tile shape and the
+ * threads per site. */
 int qcbf_autotune(qcbf_flow* h, double eps, int trial_steps, long* mode, long* win);
 
 /* ---- ensembles (no counterpart in the reference: its callers flow one
@@ -245,8 +252,9 @@ int qcbf_batch_synchronize(qcbf_batch* b);
 int qcbf_batch_tune(qcbf_batch* b, int key, long value);
 int qcbf_batch_get_tune(const qcbf_batch* b, int key, long* value);
 int qcbf_batch_use_graph(qcbf_batch* b, int on);
-/* Autotuner: tile order x persistent-grid size x tile shape, timed on the real
- * fields (saved first, restored afterwards); the fastest is kept. */
+/* Autotuner: tile order x persistent-grid size x tile shape at 4 threads per
+ * site, then 2 and 1 thread per site on the fastest of those, timed on the
+ * real fields (saved first, restored afterwards); the fastest is kept. */
 int qcbf_batch_autotune(qcbf_batch* b, double eps, int trial_steps, long* mode, long* win,
                         long* blocks_per_cu);
 /* Returns the device time of the flow_measure calls since the previous call

@@ -23,6 +23,26 @@ static int check_order(unsigned nt, unsigned per_t, int mode, unsigned win) {
     return bad;
 }
 
+// Stage workgroups with dpw directions per wavefront take a group of dpw
+// tiles: over the grid and the four wavefronts of each workgroup, every tile
+// must be reached by exactly 4/dpw wavefronts (one per block of directions).
+static int check_groups(unsigned nt, unsigned per_t, int mode, unsigned win, unsigned dpw) {
+    std::vector<int> seen(nt, 0);
+    const tile_groups tg = group_tiles(nt, per_t, mode, win, dpw);
+    const unsigned G = grid_for_tiles(tg.n, mode, tg.win, tg.per_t);
+    for (unsigned b = 0; b < G; b++)
+        for (unsigned y = 0; y < 4; y++) {
+            const unsigned t = tile_in_group(tile_of_block(b, tg.n, mode, tg.win, tg.per_t), y, dpw);
+            if (t < nt) seen[t]++;
+        }
+    int bad = 0;
+    for (unsigned t = 0; t < nt; t++) bad += seen[t] != (int)(4 / dpw);
+    if (bad)
+        std::printf("groups of %u, mode %d win %u per_t %u ntiles %u: %d tiles missed or repeated\n",
+                    dpw, mode, win, per_t, nt, bad);
+    return bad;
+}
+
 int main() {
     const int dims[][4] = {{32, 32, 32, 32}, {48, 48, 48, 48}, {16, 8, 8, 12}, {8, 8, 8, 8},
                            {12, 12, 12, 12}, {4, 6, 8, 10},     {6, 4, 5, 8},  {24, 24, 24, 24},
@@ -55,7 +75,10 @@ int main() {
             const unsigned pt = stage_tiles_per_t(g);
             if (pt) bad += nt % pt != 0;
             for (int mode = 0; mode <= 3; mode++)
-                for (unsigned w : wins) bad += check_order(nt, pt, mode, w);
+                for (unsigned w : wins) {
+                    bad += check_order(nt, pt, mode, w);
+                    for (unsigned dpw : {1u, 2u, 4u}) bad += check_groups(nt, pt, mode, w, dpw);
+                }
         }
         // measurement tiles: 64 consecutive sites
         lattice g = make_lattice(d);
