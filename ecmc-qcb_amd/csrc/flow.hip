@@ -20,8 +20,10 @@
 // Work mapping: a workgroup is 64 consecutive sites x 4 directions; wavefront
 // w handles direction mu = w, so every index computation inside a wavefront is
 // uniform and the four wavefronts of one workgroup share neighbour links
-// through the CU's L1.  The workgroup -> site-tile order is XCD-aware
-// (tile_of_block, lattice.hpp).
+// through the CU's L1.  With two or one threads per site (lattice::dpw) a
+// wavefront updates two or four directions in turn and the workgroup takes
+// two or four tiles (stage_directions, flow_body.hpp).  The workgroup ->
+// site-tile order is XCD-aware (tile_of_block, lattice.hpp).
 #include "flow_body.hpp"
 #include "kernels.hpp"
 #include "measure_body.hpp"
