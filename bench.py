@@ -9,9 +9,10 @@ through qcbf_flow_measure (K RK3 steps and K+1 measurements, one
 synchronisation at the end).
 
 Prints ONE JSON line:  value = site-flow-steps/s = V * K / seconds.
-The roofline object reports the dominant kernel (the fused RK3 stage
-kernel; its three instantiations are timed together with HIP events on the
-handle's stream inside the timed region).  The CPU baseline is the FP64
+The roofline object reports the dominant kernel, the plain RK3 stage
+kernel: its stage-2 and stage-3 launches (stage 1 runs fused with the
+measurement and is reported beside it) are timed with HIP events on the
+handle's stream inside the timed region.  The CPU baseline is the FP64
 oracle (a C restatement of the reference flow) on the host cores.
 
 --report precision | ensemble | large  runs one of the other workloads

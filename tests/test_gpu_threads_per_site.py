@@ -31,6 +31,7 @@ SINGLE_CASES = [
     ((6, 4, 5, 8), -1, None),             # 15 tiles: a last group of 3 (or 1) tiles
     ((3, 5, 4, 7), -1, (2, 1024)),        # 420 sites: partial last tile
     ((8, 8, 3, 4), -1, (3, 1)),           # 3 tiles per t-slice: groups straddle t
+    ((8, 8, 2, 5), -1, (0, 64)),          # lane shuffles, 10 tiles: a last group of 2
     ((16, 8, 8, 12), qcbflow.tile_code(4, 4, 2, 2), (3, 2)),  # box tiles, slab sweep
     ((16, 8, 8, 12), qcbflow.tile_code(8, 4, 2, 1), (1, 64)),
 ]
