@@ -68,10 +68,11 @@ def test_single_lattice_threads_per_site_bit_identical(prec, case):
 ENS_DIMS = [(4, 4, 4, 4), (6, 4, 5, 8), (3, 5, 4, 7), (8, 8, 8, 8), (4, 6, 2, 4)]
 
 
-def run_ensemble(prec, tps, bpc, graph, fs):
+def run_ensemble(prec, tps, bpc, graph, shape, fs):
     with qcbflow.Ensemble(ENS_DIMS, 0.02, prec) as ens:
         ens.use_graph(graph)
         ens.tune(qcbflow.TUNE_BLOCKS_PER_CU, bpc)
+        ens.tune(qcbflow.TUNE_TILE_SHAPE, shape)
         ens.tune(T, tps)
         assert ens.get_tune(T) == tps
         for c, f in enumerate(fs):
@@ -82,15 +83,18 @@ def run_ensemble(prec, tps, bpc, graph, fs):
 
 
 @pytest.mark.parametrize("prec", [qcbflow.FP64, qcbflow.FP32_MIXED])
-@pytest.mark.parametrize("bpc,graph", [(0, True), (2, False)])
-def test_ensemble_threads_per_site_bit_identical(prec, bpc, graph):
-    """Flat and persistent grids; a tile group may span two members."""
-    say(f"ensemble of {len(ENS_DIMS)}, precision {prec}, blocks/CU {bpc}, graph {graph}")
+@pytest.mark.parametrize("bpc,graph,shape", [(0, True, -1), (2, False, -1),
+                                             (0, False, qcbflow.tile_code(4, 4, 2, 2))])
+def test_ensemble_threads_per_site_bit_identical(prec, bpc, graph, shape):
+    """Flat and persistent grids; a tile group may span two members; a box
+    tile shape taken by the members it divides (4^4, 8^4), not by the others."""
+    say(f"ensemble of {len(ENS_DIMS)}, precision {prec}, blocks/CU {bpc}, graph {graph}, "
+        f"tile shape {shape}")
     fs = [field_for(L, 70 + k) for k, L in enumerate(ENS_DIMS)]
-    ref = run_ensemble(prec, 4, bpc, graph, fs)
+    ref = run_ensemble(prec, 4, bpc, graph, shape, fs)
     for tps in (2, 1):
         say(f"  threads per site {tps}")
-        got = run_ensemble(prec, tps, bpc, graph, fs)
+        got = run_ensemble(prec, tps, bpc, graph, shape, fs)
         assert np.array_equal(ref[0], got[0]) and np.array_equal(ref[1], got[1]), tps
         for a, b in zip(ref[2], got[2]):
             assert np.array_equal(a, b), tps
