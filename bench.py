@@ -585,7 +585,9 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_rk3_stage<f64,1|2|3> (fused RK3 stage)",
+            "kernel": "k_rk3_stage<f64, " + "|".join(str(k + 1) for k in range(3) if launches[k])
+                      + "> (one whole RK3 stage per launch"
+                      + ("; stage 1 runs fused with the measurement)" if launches[4] else ")"),
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

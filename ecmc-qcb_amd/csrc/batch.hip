@@ -269,18 +269,12 @@ __global__ __launch_bounds__(256, 2) void k_batch_stage1_measure(
 }
 
 // One workgroup per configuration: ordered, compensated sum of its tiles.
-__global__ __launch_bounds__(256) void k_batch_reduce(const double* __restrict__ partials,
-                                                      const cfg_desc* __restrict__ cfgs, int ncfg,
-                                                      double* __restrict__ out,
-                                                      const unsigned* __restrict__ slot_ctr) {
-    __shared__ double lds[4 * 3];
+__global__ __launch_bounds__(kReduceThreads) void k_batch_reduce(
+    const double* __restrict__ partials, const cfg_desc* __restrict__ cfgs, int ncfg,
+    double* __restrict__ out, const unsigned* __restrict__ slot_ctr) {
     const cfg_desc& c = cfgs[blockIdx.x];
-    double v[3] = {0.0, 0.0, 0.0}, comp[3] = {0.0, 0.0, 0.0};
-    strided_compensated_sum<3>(partials + 3 * (size_t)c.tile0, c.ntiles, threadIdx.x, blockDim.x, v,
-                               comp);
-#pragma unroll
-    for (int k = 0; k < 3; k++) v[k] += comp[k];
-    block_sum<3>(v, lds, 4);
+    double v[3];
+    tile_triples_sum(partials + 3 * (size_t)c.tile0, c.ntiles, v);
     if (threadIdx.x == 0) {
         const unsigned slot = slot_ctr ? *slot_ctr : 0u;
         double* o = out + 3 * ((size_t)slot * ncfg + blockIdx.x);
@@ -321,7 +315,7 @@ hipError_t launch_batch_measure(prec p, const void* U, const cfg_desc* cfgs,
     else
         k_batch_measure<prec::f32m, false><<<nslots, block, 0, st>>>(
             u32, cfgs, tile_cfg, ntiles, bl.map_mode, bl.map_win, partials);
-    k_batch_reduce<<<ncfg, 256, 0, st>>>(partials, cfgs, ncfg, out, slot_ctr);
+    k_batch_reduce<<<ncfg, kReduceThreads, 0, st>>>(partials, cfgs, ncfg, out, slot_ctr);
     if (bump && slot_ctr) k_bump<<<1, 1, 0, st>>>(slot_ctr);
     return hipGetLastError();
 }
@@ -356,7 +350,7 @@ hipError_t launch_batch_stage1_measure(prec p, const void* Uin, void* Uout, void
             k_batch_stage1_measure<prec::f32m, false><<<grid, block, 0, st>>>(
                 in, out, a, cfgs, tile_cfg, ntiles, bl.map_mode, bl.map_win, cz, warn, partials);
     }
-    k_batch_reduce<<<ncfg, 256, 0, st>>>(partials, cfgs, ncfg, out, slot_ctr);
+    k_batch_reduce<<<ncfg, kReduceThreads, 0, st>>>(partials, cfgs, ncfg, out, slot_ctr);
     if (bump && slot_ctr) k_bump<<<1, 1, 0, st>>>(slot_ctr);
     return hipGetLastError();
 }

@@ -100,15 +100,10 @@ __global__ __launch_bounds__(192) void k_plaquette(const typename storage<P>::ve
 // the reference's normalisations: P = sum Re tr / 3 / (6V)
 // (observables_mpi.cpp:48,67), E = sum e / V (:173-176), Q = sum q / (4 pi^2)
 // (:132).
-__global__ __launch_bounds__(256) void k_reduce_measure(const double* __restrict__ partials,
-                                                        unsigned nblk, double invV,
-                                                        double* __restrict__ out) {
-    __shared__ double lds[4 * 3];
-    double v[3] = {0.0, 0.0, 0.0}, c[3] = {0.0, 0.0, 0.0};
-    strided_compensated_sum<3>(partials, nblk, threadIdx.x, blockDim.x, v, c);
-#pragma unroll
-    for (int k = 0; k < 3; k++) v[k] += c[k];
-    block_sum<3>(v, lds, 4);
+__global__ __launch_bounds__(kReduceThreads) void k_reduce_measure(
+    const double* __restrict__ partials, unsigned nblk, double invV, double* __restrict__ out) {
+    double v[3];
+    tile_triples_sum(partials, nblk, v);
     if (threadIdx.x == 0) {
         out[0] = v[0] / 3.0 / 6.0 * invV;
         out[1] = v[1] * invV;
@@ -126,14 +121,14 @@ static hipError_t measure_launch(const void* U, const lattice& g, double* partia
         k_measure_clover<P><<<grid, dim3(kSitesPerBlock, 4), 0, st>>>(u, g, partials);
     else
         k_plaquette<P><<<grid, dim3(kSitesPerBlock, 3), 0, st>>>(u, g, partials);
-    k_reduce_measure<<<1, 256, 0, st>>>(partials, nblk, 1.0 / (double)g.V, out);
+    k_reduce_measure<<<1, kReduceThreads, 0, st>>>(partials, nblk, 1.0 / (double)g.V, out);
     return hipGetLastError();
 }
 
 hipError_t launch_reduce_measure(const double* partials, const lattice& g, double* out,
                                  hipStream_t st) {
     (void)hipGetLastError();  // drop a stale error of an earlier call
-    k_reduce_measure<<<1, 256, 0, st>>>(partials, measure_blocks(g), 1.0 / (double)g.V, out);
+    k_reduce_measure<<<1, kReduceThreads, 0, st>>>(partials, measure_blocks(g), 1.0 / (double)g.V, out);
     return hipGetLastError();
 }
 
