@@ -277,11 +277,7 @@ __global__ __launch_bounds__(kReduceThreads) void k_batch_reduce(
     tile_triples_sum(partials + 3 * (size_t)c.tile0, c.ntiles, v);
     if (threadIdx.x == 0) {
         const unsigned slot = slot_ctr ? *slot_ctr : 0u;
-        double* o = out + 3 * ((size_t)slot * ncfg + blockIdx.x);
-        const double invV = 1.0 / (double)c.g.V;
-        o[0] = v[0] / 3.0 / 6.0 * invV;
-        o[1] = v[1] * invV;
-        o[2] = v[2] * (1.0 / (4.0 * M_PI * M_PI));
+        write_observables(v, 1.0 / (double)c.g.V, out + 3 * ((size_t)slot * ncfg + blockIdx.x));
     }
 }
 

@@ -97,18 +97,12 @@ __global__ __launch_bounds__(192) void k_plaquette(const typename storage<P>::ve
 }
 
 // Ordered, Neumaier-compensated reduction of the per-workgroup partials and
-// the reference's normalisations: P = sum Re tr / 3 / (6V)
-// (observables_mpi.cpp:48,67), E = sum e / V (:173-176), Q = sum q / (4 pi^2)
-// (:132).
+// the reference's normalisations (tile_triples_sum, write_observables).
 __global__ __launch_bounds__(kReduceThreads) void k_reduce_measure(
     const double* __restrict__ partials, unsigned nblk, double invV, double* __restrict__ out) {
     double v[3];
     tile_triples_sum(partials, nblk, v);
-    if (threadIdx.x == 0) {
-        out[0] = v[0] / 3.0 / 6.0 * invV;
-        out[1] = v[1] * invV;
-        out[2] = v[2] * (1.0 / (4.0 * M_PI * M_PI));
-    }
+    if (threadIdx.x == 0) write_observables(v, invV, out);
 }
 
 template <prec P>
