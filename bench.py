@@ -35,6 +35,10 @@ METRIC = "site·flow-steps/s, FP32-mixed vs FP64 GPU at |dE/E|<1e-6; HBM GB/s vs
 UNIT = "site·flow-steps/s"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+# MI355X's published FP32 vector peak counts packed (two-wide) FP32 issue; this
+# build compiles with packed FP32 switched off, which halves the issue rate.
+FP32_PEAK_TFLOPS = 157.3
+FP32_UNPACKED_TFLOPS = 78.6
 
 # Algorithmic work per site per RK3 step (FP64 path):
 #   bytes: each stage reads and writes the 4 links (144 B each) and the su(3)
@@ -543,6 +547,8 @@ def main():
             "launch_shape": shape32,
             "accuracy_vs_fp64": compare(tqe, plaq, mtqe, mplaq),
         }
+        mixed["fp32_frac"] = mixed["fp32_tflops"] / FP32_PEAK_TFLOPS
+        mixed["fp32_frac_unpacked"] = mixed["fp32_tflops"] / FP32_UNPACKED_TFLOPS
         log(f"fp32-mixed: {mixed['value']:.4e} site-steps/s ({mixed['vs_fp64']:.2f}x fp64), "
             f"stage {mstage:.4f} ms, dE/E at t=2 {mixed['accuracy_vs_fp64']['dE_rel_at_t_gate']:.2e}")
         gm.close()
