@@ -143,19 +143,32 @@ inline lattice make_lattice(const int dims[4]) {
         g.ntile[d] = 0;
     }
     g.dpw = 1;
-    // Slab sweep with 16 slabs per t-slice where the tiles allow it, else XCD
-    // chunks in windows of 1024 (measured on MI355X at 32^4, 48^4 and 64^4;
-    // qcbf_autotune refines it).
+    // Slab sweep with 16 slabs per t-slice, or as many more (32 ... 512) as
+    // keep a slab at most 256 tiles: at 96^4 a 16-slab slab is 864 tiles,
+    // whose links at t-1, t and t+1 outgrow the caches, and 216-tile slabs ran
+    // 9% faster (FP64).  Equal slabs first; then slabs of at most 256 tiles
+    // with a shorter last one (the order pads it); then 16 or 8 slabs of any
+    // size; else XCD chunks in windows of 1024 (measured on MI355X at 32^4,
+    // 48^4, 64^4 and 96^4; qcbf_autotune refines it).
     const unsigned pt = site_tiles_per_t(g);
-    if (pt && pt % 16 == 0) {
+    g.map_mode = 2;
+    g.map_win = 1024;
+    for (unsigned ns : {16u, 32u, 64u, 128u, 256u, 512u})
+        if (pt && pt % ns == 0 && pt / ns <= 256u) {
+            g.map_mode = 3;
+            g.map_win = pt / ns;
+            break;
+        }
+    if (g.map_mode == 2 && pt > 16u * 256u)
+        for (unsigned ns : {32u, 64u, 128u, 256u, 512u})
+            if ((pt + ns - 1) / ns <= 256u) {
+                g.map_mode = 3;
+                g.map_win = (pt + ns - 1) / ns;
+                break;
+            }
+    if (g.map_mode == 2 && pt && (pt % 16 == 0 || pt % 8 == 0)) {
         g.map_mode = 3;
-        g.map_win = pt / 16;
-    } else if (pt && pt % 8 == 0) {
-        g.map_mode = 3;
-        g.map_win = pt / 8;
-    } else {
-        g.map_mode = 2;
-        g.map_win = 1024;
+        g.map_win = pt % 16 == 0 ? pt / 16 : pt / 8;
     }
     return g;
 }

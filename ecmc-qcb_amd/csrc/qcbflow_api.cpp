@@ -891,9 +891,9 @@ extern "C" int qcbf_autotune(qcbf_flow* h, double eps, int trial_steps, long* mo
         if (sh[0] >= 0 && !qcb::set_tile_shape(probe, sh)) continue;
         if (sh[0] < 0) qcb::set_tile_shape(probe, nullptr);
         for (const auto& m : maps) cands.push_back({m.first, m.second, {sh[0], sh[1], sh[2], sh[3]}, 1});
-        // slab sweeps: 8, 16 or 32 slabs per t-slice (one, two or four per XCD)
+        // slab sweeps: 8 to 128 slabs per t-slice (one to sixteen per XCD)
         const unsigned pt = qcb::stage_tiles_per_t(probe);
-        for (unsigned ns : {8u, 16u, 32u})
+        for (unsigned ns : {8u, 16u, 32u, 64u, 128u})
             if (pt && pt % ns == 0) cands.push_back({3, pt / ns, {sh[0], sh[1], sh[2], sh[3]}, 1});
     }
     const size_t first_pass = cands.size();

@@ -190,8 +190,10 @@ enum qcbf_tune_key {
     QCBF_TUNE_MAP_WIN = 1,  /* mode 2: window (rounded down to a multiple of 8, at
                                least 8); mode 3: slab width in tiles; >= 1.  With
                                fewer than 8 slabs per t-slice some XCDs get no
-                               slab (32^4: MAP_WIN 128 leaves half of them idle);
-                               the default and qcbf_autotune keep 8 to 32 slabs */
+                               slab (32^4: MAP_WIN 128 leaves half of them idle).
+                               The default takes 16 slabs per t-slice, more when
+                               that keeps slabs at most 256 tiles (96^4: 64);
+                               qcbf_autotune tries 8 to 128 */
     QCBF_TUNE_BLOCKS_PER_CU = 2, /* ensembles only: persistent-grid workgroups per CU
                                     (1..16), or 0 (default) for one workgroup per tile */
     QCBF_TUNE_FUSED_MEASURE = 4, /* qcbf_flow_measure fuses each measurement into the

@@ -282,6 +282,52 @@ def test_largest_lattice_properties_64x64x64x64():
         assert abs(em - ea) < 1e-6 * ea and abs(qm - qa) < 1e-4
 
 
+def test_lattice_of_120_gb_96x96x96x96():
+    """A 96^4 lattice (84.9M sites, 2.7e9 link words past 2^31) on one GPU: an
+    FP64 handle keeps 120 GB resident, an FP32-mixed one 43.5 GB more.  On the
+    device only (a host copy of the field would be 49 GB): unitarity after two
+    RK3 steps, P, E, Q unchanged by a periodic shift, the flow of the shifted
+    field measuring as the flow of the field, and the FP32-mixed gates."""
+    L = (96, 96, 96, 96)
+    s = (95, 7, 48, 33)
+    with qcbflow.GradientFlow(L, 0.02) as a:
+        say("96^4: hot start, P, E, Q before and after a periodic shift")
+        a.hot_start(96)
+        m0 = a.measure()
+        a.shift(s)
+        m0s = a.measure()
+        say(f"  P {m0[0]:.15f} E {m0[1]:.12g} Q {m0[2]:.6g}; shifted: dP {abs(m0s[0] - m0[0]):.1e} "
+            f"dE {abs(m0s[1] - m0[1]):.1e} dQ {abs(m0s[2] - m0[2]):.1e}")
+        # a shift regroups the site sums: rounding only (Q sums 85M rough terms)
+        assert abs(m0s[0] - m0[0]) < 1e-13 and abs(m0s[1] - m0[1]) < 1e-12 * m0[1]
+        assert abs(m0s[2] - m0[2]) < 1e-8
+        say("96^4: 2 RK3 steps, unitarity")
+        a.hot_start(96)
+        a.rk3_step(2)
+        du, dd = a.unitarity()
+        assert du < 1e-13 and dd < 1e-13
+        m1 = a.measure()
+        say("96^4: 2 RK3 steps of the shifted field")
+        a.hot_start(96)
+        a.shift(s)
+        a.rk3_step(2)
+        m2 = a.measure()
+        say(f"  P {m1[0]:.15f} E {m1[1]:.12g} Q {m1[2]:.6g}; shifted: dP {abs(m2[0] - m1[0]):.1e} "
+            f"dE {abs(m2[1] - m1[1]):.1e} dQ {abs(m2[2] - m1[2]):.1e}")
+        assert abs(m2[0] - m1[0]) < 1e-13 and abs(m2[1] - m1[1]) < 1e-12 * m1[1]
+        assert abs(m2[2] - m1[2]) < 1e-8
+        say("96^4: FP32-mixed against FP64, 2 RK3 steps")
+        with qcbflow.GradientFlow(L, 0.02, qcbflow.FP32_MIXED) as m:
+            a.hot_start(96)
+            m.copy_from(a)
+            a.rk3_step(2)
+            m.rk3_step(2)
+            pa, ea, qa = a.measure()
+            pm, em, qm = m.measure()
+            say(f"  dE/E {abs(em - ea) / ea:.2e}, dQ {abs(qm - qa):.2e}, dP/P {abs(pm - pa) / pa:.2e}")
+            assert abs(em - ea) < 1e-6 * ea and abs(qm - qa) < 1e-4
+
+
 @pytest.mark.parametrize("prec", [qcbflow.FP64, qcbflow.FP32_MIXED])
 def test_flow_commutes_with_shift_48x48x48x48(prec):
     """The 48^4 configuration (x-extent 48: the FP64 kernels load the
