@@ -246,7 +246,7 @@ def test_large_lattice_properties_32x32x32x32():
         assert abs(p3 - p0) < 1e-13 and abs(e3 - e0) < 1e-12 * e0 and abs(q3 - q0) < 1e-10
 
 
-def test_largest_lattice_properties_64x64x64x64():
+def test_lattice_properties_64x64x64x64():
     """The 64^4 configuration (16.8M sites, 23.6 GB per FP64 handle), on the
     device only: unitarity after two RK3 steps, P, E, Q unchanged by a
     periodic shift, the flow commuting with the shift bit for bit (every link
