@@ -46,8 +46,10 @@ struct qcbf_flow {
     unsigned hb_sweeps = 0;           // heatbath sweeps done (keys the random streams)
     void* stage = nullptr;            // device staging chunk for host <-> HBM transfers
     std::string err;
-    // timing
+    // timing: every timing_every-th launch of each kernel class is bracketed
     bool timing = false;
+    int timing_every = 1;
+    long launch_no[QCBF_K_COUNT] = {};
     std::vector<qcbf_event> events;
     size_t ev_used = 0;
     double tot_ms[QCBF_K_COUNT] = {};
@@ -105,7 +107,7 @@ static int timing_flush(qcbf_flow* h) {
 }
 static int timing_begin(qcbf_flow* h, int cls, qcbf_event** ev) {
     *ev = nullptr;
-    if (!h->timing) return QCBF_OK;
+    if (!h->timing || h->launch_no[cls]++ % h->timing_every != 0) return QCBF_OK;
     if (h->ev_used == h->events.size()) {
         int rc = timing_flush(h);
         if (rc) return rc;
@@ -770,6 +772,7 @@ extern "C" int qcbf_get_tune(const qcbf_flow* h, int key, long* value) {
 // ---- measurement support ------------------------------------------------------------
 extern "C" int qcbf_timing_enable(qcbf_flow* h, int on) {
     CHECK_H(h);
+    if (on < 0) return fail(h, QCBF_ERR_ARG, "timing_enable: %d < 0", on);
     int rc = timing_flush(h);
     if (rc) return rc;
     if (on && h->events.empty()) {
@@ -781,9 +784,11 @@ extern "C" int qcbf_timing_enable(qcbf_flow* h, int on) {
         }
     }
     h->timing = on != 0;
+    h->timing_every = on > 0 ? on : 1;
     for (int k = 0; k < QCBF_K_COUNT; k++) {
         h->tot_ms[k] = 0.0;
         h->cnt[k] = 0;
+        h->launch_no[k] = 0;
     }
     return QCBF_OK;
 }

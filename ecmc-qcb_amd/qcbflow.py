@@ -328,7 +328,10 @@ class GradientFlow:
 
     # -- measurement support
     def timing(self, on=True):
-        self._c(load().qcbf_timing_enable(self._h, 1 if on else 0))
+        """Time the kernel launches of this handle: True (every launch), False
+        (off), or an int n >= 1 (every n-th launch of each kernel class)."""
+        n = 1 if on is True else 0 if on is False else int(on)
+        self._c(load().qcbf_timing_enable(self._h, n))
 
     def timing_read(self):
         ms = (ctypes.c_double * K_COUNT)()

@@ -264,9 +264,12 @@ int qcbf_batch_autotune(qcbf_batch* b, double eps, int trial_steps, long* mode, 
 int qcbf_batch_timing(qcbf_batch* b, int on, double* total_ms, long* rk3_steps);
 
 /* ---- measurement support (no counterpart in the reference) -------------- */
-/* When on, every fused-stage and measurement launch on h is bracketed by HIP
- * events on h's stream; qcbf_timing_read synchronises and returns, per kernel
- * class, the summed kernel milliseconds and launch count since timing was
+/* With on = n >= 1, every n-th launch of each kernel class on h (the first,
+ * the (n+1)-th, ...) is bracketed by HIP events on h's stream; on = 0 stops.
+ * n = 1 times every launch.  A larger n keeps most of the event markers, which
+ * add a few microseconds between kernels, out of a timed loop.
+ * qcbf_timing_read synchronises and returns, per kernel class, the summed
+ * milliseconds and the number of timed launches since timing was
  * (re)enabled. */
 int qcbf_timing_enable(qcbf_flow* h, int on);
 int qcbf_timing_read(qcbf_flow* h, double total_ms[QCBF_K_COUNT], long launches[QCBF_K_COUNT]);

@@ -415,3 +415,24 @@ def test_fused_measurement_is_bit_identical(prec, L):
         ta2, pa2 = a.flow_measure(4, 2)  # measurement every second step
         tb2, pb2 = b.flow_measure(4, 2)
         assert np.array_equal(ta2, tb2) and np.array_equal(pa2, pb2)
+
+
+def test_timing_samples_every_nth_launch():
+    """qcbf_timing_enable(h, n): the first, (n+1)-th, ... launch of each kernel
+    class is timed; n = 1 times all of them; a negative n is refused."""
+    L = (8, 8, 8, 8)
+    with qcbflow.GradientFlow(L) as gf:
+        gf.hot_start(3)
+        for n, want in ((1, 7), (3, 3), (4, 2), (8, 1)):
+            gf.timing(n)
+            gf.rk3_step(7)
+            ms, launches = gf.timing_read()
+            assert launches[:3] == [want] * 3, (n, launches)
+            assert all(m > 0 for m in ms[:3])
+        gf.timing(2)
+        gf.flow_measure(5, 1)  # 4 fused stage-1 launches, stages 2 and 3, 1 measurement
+        ms, launches = gf.timing_read()
+        assert launches == [0, 2, 2, 1, 2], launches
+        gf.timing(False)
+        with pytest.raises(qcbflow.QcbfError):
+            gf.timing(-1)
