@@ -43,6 +43,20 @@ static int check_groups(unsigned nt, unsigned per_t, int mode, unsigned win, uns
     return bad;
 }
 
+// The default tile order of make_lattice: slab sweeps of at most 256 tiles
+// where the t-slice allows it, the known choices at the benchmarked sizes.
+static int check_default(int L, int mode, unsigned win) {
+    const int d[4] = {L, L, L, L};
+    const lattice g = make_lattice(d);
+    const unsigned pt = site_tiles_per_t(g);
+    int bad = g.map_mode != mode || g.map_win != win;
+    if (g.map_mode == 3 && pt > 16u * 256u) bad += g.map_win > 256u;
+    if (bad)
+        std::printf("default order %d^4: mode %d win %u, expected %d %u\n", L, g.map_mode,
+                    g.map_win, mode, win);
+    return bad;
+}
+
 int main() {
     const int dims[][4] = {{32, 32, 32, 32}, {48, 48, 48, 48}, {16, 8, 8, 12}, {8, 8, 8, 8},
                            {12, 12, 12, 12}, {4, 6, 8, 10},     {6, 4, 5, 8},  {24, 24, 24, 24},
@@ -86,6 +100,11 @@ int main() {
         for (int mode = 0; mode <= 3; mode++)
             for (unsigned w : wins) bad += check_order(nm, pt, mode, w);
     }
+    // 8^4 .. 64^4 keep 16 (or 8) slabs; 96^4, 104^4 and 112^4 get more, smaller ones
+    bad += check_default(8, 3, 1) + check_default(12, 2, 1024) + check_default(16, 3, 4) +
+           check_default(24, 3, 27) + check_default(32, 3, 32) + check_default(48, 3, 108) +
+           check_default(64, 3, 256) + check_default(96, 3, 216) + check_default(104, 3, 138) +
+           check_default(112, 3, 172);
     std::printf("checked %d lattice/shape pairs, %d errors\n", tried, bad);
     return bad != 0;
 }
