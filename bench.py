@@ -1,26 +1,30 @@
 """Throughput of the Wilson-flow measurement path on one MI355X.
 
-Workload (default): a 32^4 SU(3) lattice in FP64, Haar hot start from seed 7
-thermalised by 50 checkerboard heatbath sweeps at beta = 6.0 on the GPU, then
-Wilson flow at eps = 0.02 with the clover Q, E and the plaquette measured
-after every RK3 step.  One bench "step" = one RK3 step (three fused stage
-kernels) + one measurement.  The timed region runs K steps (default 200)
-through qcbf_flow_measure (K RK3 steps and K+1 measurements, one
-synchronisation at the end).
+Workload (default): the large single lattice, 64^4 SU(3) (16.8M sites), Haar
+hot start from seed 2026 thermalised by 50 checkerboard heatbath sweeps at
+beta = 6.4 on the GPU, then Wilson flow at eps = 0.02 with the clover Q, E
+and the plaquette measured every 10 RK3 steps.  One bench "step" is one RK3
+step (three stage kernels); the timed region runs K steps (default 400)
+through qcbf_flow_measure (K/10 + 1 measurements, one synchronisation at the
+end).  `--preset c2` runs the 32^4 lattice (beta 6.0, seed 7, 50 sweeps)
+measured after every step instead.
 
-Prints ONE JSON line:  value = site-flow-steps/s = V * K / seconds.
-The roofline object reports the dominant kernel, the plain RK3 stage
-kernel: its stage-2 and stage-3 launches (stage 1 runs fused with the
-measurement and is reported beside it) are timed with HIP events on the
-handle's stream inside the timed region.  The CPU baseline is the FP64
-oracle (a C restatement of the reference flow) on the host cores.
+Prints ONE JSON line:  value = FP64 site-flow-steps/s = V * K / seconds.
+Beside it: the FP32-mixed form on the same input and cadence (`fp32_mixed`),
+its accuracy against FP64 at flow time t = 2 from a separate untimed leg of
+2/eps steps in both precisions (whatever K is), the roofline object of the
+dominant kernel (the plain RK3 stage kernel, every launch of the timed
+region bracketed by HIP events on the handle's stream) and the CPU baseline
+(the FP64 oracle, a C restatement of the reference flow, on the host cores:
+one RK3 step and one measurement of the same field, extrapolated).
 
---report precision | ensemble | large  runs one of the other workloads
-instead (48^4 FP32-mixed vs FP64 accuracy and t0; the 64-lattice ensemble of
-uneven volumes; 64^4 FP32-mixed vs FP64) and prints one JSON line for it.
+--report precision | ensemble  runs one of the other workloads instead
+(48^4 FP32-mixed vs FP64 accuracy and t0; the 64-lattice ensemble of uneven
+volumes) and prints one JSON line for it.
 """
 import argparse
 import json
+import math
 import os
 import re
 import sys
@@ -58,6 +62,13 @@ STAGE_BYTES = {"fp64": (1408.0, 1664.0, 1408.0), "fp32_mixed": (512.0, 640.0, 51
 # one read of the links by the measurement
 MEASURE_BYTES = {"fp64": 576.0, "fp32_mixed": 192.0}
 
+# Single-lattice workloads: (lattice extent, beta, seed, heatbath sweeps,
+# RK3 steps between measurements, default timed steps).
+PRESETS = {
+    "large": (64, 6.4, 2026, 50, 10, 400),  # 64^4, measured every 10 steps
+    "c2": (32, 6.0, 7, 50, 1, 200),         # 32^4, measured every step
+}
+
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
@@ -65,7 +76,7 @@ def log(msg):
 
 def traffic_from_profiles(lattice):
     """HBM bytes per stage launch from the committed rocprofv3 PMC summary of
-    this workload, or None."""
+    this workload (profiles/rk3_stage_f64_<lattice>_traffic.md), or None."""
     tag = "x".join(str(v) for v in lattice)
     path = os.path.join(ROOT, "profiles", f"rk3_stage_f64_{tag}_traffic.md")
     if not os.path.exists(path):
@@ -75,59 +86,70 @@ def traffic_from_profiles(lattice):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(links, L, eps, steps, threads):
-    """The FP64 oracle (C, OpenMP) on the same 32^4 field: `steps` RK3 steps,
-    each followed by one measurement."""
+def host_cpus():
+    """CPUs this process may run on (nproc) and the host CPU model (lscpu's
+    'Model name', from /proc/cpuinfo)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return n, model
+
+
+def cpu_baseline(links, L, eps, every, threads, steps=1):
+    """The FP64 oracle (C, OpenMP) on the same field: `steps` RK3 steps and one
+    measurement, each timed, combined at the workload's cadence (one
+    measurement every `every` steps) into site-flow-steps/s."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
     oracle.set_num_threads(threads)
     f = links.copy()
-    secs = oracle.time_flow(f, L, eps, steps, measure=True)
-    V = 1
-    for v in L:
-        V *= v
+    secs_flow = oracle.time_flow(f, L, eps, steps, measure=False)
+    t0 = time.perf_counter()
+    oracle.q_e_clover(f, L)
+    oracle.plaquette(f, L)
+    secs_meas = time.perf_counter() - t0
+    del f
+    V = volume(L)
+    per_step = secs_flow / steps + secs_meas / every
+    nproc, model = host_cpus()
     return {
-        "value": V * steps / secs,
+        "value": V / per_step,
         "unit": UNIT,
         "cores": threads,
         "kind": "port",
-        "sample": f"{steps} RK3 steps + {steps} measurements of the same "
-                  f"{'x'.join(map(str, L))} FP64 field, {secs:.2f} s, "
-                  f"OpenMP {threads} threads of {os.cpu_count()} ({cpu_model()}), "
-                  "C oracle, gcc -O2 -fopenmp",
+        "sample": f"{steps} RK3 step(s) ({secs_flow:.2f} s) and 1 measurement ({secs_meas:.2f} s) "
+                  f"of the same {'x'.join(map(str, L))} FP64 field, extrapolated to the "
+                  f"workload's cadence (1 measurement per {every} steps); OpenMP {threads} "
+                  f"threads, nproc {nproc} ({model}); C oracle, gcc -O2 -fopenmp",
+        "rk3_step_s": secs_flow / steps,
+        "measure_s": secs_meas,
+        "nproc": nproc,
+        "cpu_model": model,
     }
 
 
-def cpu_model():
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
-    except OSError:
-        pass
-    return "unknown CPU"
-
-
-# The kernel durations of the timed region come from HIP events around every
-# TIME_EVERY-th launch of each kernel class.  Bracketing every launch put
-# about 15 us (FP64) and 19 us (FP32-mixed) of event markers between the
-# kernels of each 32^4 RK3 step, 1-2% of the throughput being measured.
-TIME_EVERY = 4
-
-
-def timed_flow(h, src, K, warmup, every=1):
+def timed_flow(h, src, K, warmup, every=1, time_every=1):
     """Warmup on the handle, then refill it from `src` (a handle holding the
     input field) so the timed flow starts at t = 0 on the input, and run
     K RK3 steps with a measurement every `every` steps (K divisible by it).
     Returns (seconds, tQE, plaquettes, per-class kernel ms, per-class launches);
-    the kernel times cover every TIME_EVERY-th launch of each class."""
+    the kernel times cover every `time_every`-th launch of each class."""
     if warmup > 0:
         h.flow_measure(warmup + 1, 1)
     h.copy_from(src)
     h.synchronize()
-    h.timing(TIME_EVERY)
+    h.timing(time_every)
     h.synchronize()
     t0 = time.perf_counter()
     tqe, plaq = h.flow_measure(K // every + 1, every)
@@ -160,34 +182,61 @@ def t_at(tqe, target):
     return float(t[k - 1] + (target - y[k - 1]) * (t[k] - t[k - 1]) / (y[k] - y[k - 1]))
 
 
-def row_at(tqe, t):
-    return int(np.argmin(np.abs(tqe[:, 0] - t)))
-
-
 def compare(t64, p64, t32, p32, t_gate=2.0):
     """Errors of the FP32-mixed series against the FP64 series (same input,
-    same cadence): at t = t_gate and maxima over the whole flow."""
-    k = row_at(t64, t_gate)
+    same cadence): at flow time t_gate and maxima over the whole flow.  The
+    gates are None unless the series holds a measurement at t_gate."""
+    k = int(np.argmin(np.abs(t64[:, 0] - t_gate)))
+    at_gate = abs(t64[k, 0] - t_gate) < 1e-9
     dE = np.abs(t32[:, 2] - t64[:, 2]) / np.abs(t64[:, 2])
     dQ = np.abs(t32[:, 1] - t64[:, 1])
     dP = np.abs(p32 - p64) / np.abs(p64)
     return {
-        "t_gate": float(t64[k, 0]),
-        "dE_rel_at_t_gate": float(dE[k]),
-        "dQ_abs_at_t_gate": float(dQ[k]),
-        "dP_rel_at_t_gate": float(dP[k]),
+        "t_gate": float(t64[k, 0]) if at_gate else None,
+        "dE_rel_at_t_gate": float(dE[k]) if at_gate else None,
+        "dQ_abs_at_t_gate": float(dQ[k]) if at_gate else None,
+        "dP_rel_at_t_gate": float(dP[k]) if at_gate else None,
+        "E_fp64_at_t_gate": float(t64[k, 2]) if at_gate else None,
+        "Q_fp64_at_t_gate": float(t64[k, 1]) if at_gate else None,
         "max_dE_rel": float(dE.max()),
         "max_dQ_abs": float(dQ.max()),
         "max_dP_rel": float(dP.max()),
-        "gate_dE_lt_1e-6": bool(dE[k] < 1e-6),
-        "gate_dQ_lt_1e-4": bool(dQ[k] < 1e-4),
+        "gate_dE_lt_1e-6": bool(dE[k] < 1e-6) if at_gate else None,
+        "gate_dQ_lt_1e-4": bool(dQ[k] < 1e-4) if at_gate else None,
     }
 
 
+def accuracy_leg(qcbflow, src, L, eps, every, t_end=2.0):
+    """FP64 and FP32-mixed flows of the same input to t = t_end (t_end/eps RK3
+    steps, measured every `every` steps, untimed); the FP32-mixed errors
+    against FP64 at t_end, and where t^2 E first reaches 0.15 and 0.3."""
+    n = int(round(t_end / eps))
+    if n % every:
+        every = math.gcd(n, every)
+    series = {}
+    for prec in (qcbflow.FP64, qcbflow.FP32_MIXED):
+        with qcbflow.GradientFlow(L, eps, prec) as h:
+            h.copy_from(src)
+            series[prec] = h.flow_measure(n // every + 1, every)
+    (t64, p64), (t32, p32) = series[qcbflow.FP64], series[qcbflow.FP32_MIXED]
+    acc = compare(t64, p64, t32, p32, t_gate=n * eps)
+    acc["rk3_steps"] = n
+    acc["measured_every"] = every
+    # E_ref is the reference's (1/2) sum |F|^2 / V, half the usual
+    # normalisation: t^2 E_ref = 0.15 is the customary t0, 0.3 the literal one
+    t0s = {}
+    for thr in (0.15, 0.3):
+        a, b = t_at(t64, thr), t_at(t32, thr)
+        t0s[f"{thr}"] = {"fp64": a, "fp32_mixed": b,
+                         "diff": None if a is None or b is None else abs(a - b)}
+    acc["t_at_t2E"] = t0s
+    return acc
+
+
 def stage_roofline(ms, launches, V, tag):
-    """Average duration (ms) and algorithmic GB/s of the plain stage kernels
-    (timing classes 0-2; with the fused measurement stage 1 runs as class 4),
-    bytes weighted by stage."""
+    """Average duration (ms), algorithmic GB/s and average algorithmic bytes of
+    the plain stage kernels (timing classes 0-2; a stage 1 fused with the
+    measurement is class 4), bytes weighted by stage."""
     sb = STAGE_BYTES[tag]
     n = launches[0] + launches[1] + launches[2]
     t = ms[0] + ms[1] + ms[2]
@@ -226,7 +275,7 @@ def setup_shape(qcbflow, h, args):
 
 def lattice_of(args, default):
     if args.L is None:
-        return default
+        return tuple(default) if not np.isscalar(default) else (default,) * 4
     return tuple(args.L * 4) if len(args.L) == 1 else tuple(args.L)
 
 
@@ -251,11 +300,16 @@ def device_memory(V, tag):
             "fp32_18_real_field_gb": 4 * 72 * V / 1e9 if tag == "fp32_mixed" else None}
 
 
-def report_single(qcbflow, args, name, L, beta, seed, sweeps, K, every):
-    """FP64 and FP32-mixed flows of one thermalised lattice: throughput, stage
-    roofline, and the FP32-mixed errors against FP64 (P, E, Q, t0)."""
+def report_precision(qcbflow, args):
+    """48^4 at beta = 6.2 from seed 11: FP64 and FP32-mixed flows of 250 RK3
+    steps measured after every step; throughput, stage roofline, and the
+    FP32-mixed errors against FP64 (P, E, Q at t = 2, t0)."""
+    L = lattice_of(args, 48)
+    K = args.steps if args.steps_given else 250
+    beta, seed = 6.2, 11
+    sweeps = args.sweeps if args.sweeps is not None else 100
     V = volume(L)
-    log(f"{name}: lattice {L}, V = {V}, beta {beta}, seed {seed}, {sweeps} heatbath sweeps")
+    log(f"precision: lattice {L}, V = {V}, beta {beta}, seed {seed}, {sweeps} heatbath sweeps")
     t0 = time.perf_counter()
     src = thermalised(qcbflow, L, beta, seed, sweeps)
     log(f"  thermalised in {time.perf_counter() - t0:.2f} s, plaquette {src.plaquette():.6f}")
@@ -266,7 +320,7 @@ def report_single(qcbflow, args, name, L, beta, seed, sweeps, K, every):
         h = qcbflow.GradientFlow(L, args.eps, prec)
         h.copy_from(src)
         shape = setup_shape(qcbflow, h, args)
-        secs, tqe, plaq, ms, n = timed_flow(h, src, K, args.warmup, every)
+        secs, tqe, plaq, ms, n = timed_flow(h, src, K, args.warmup, 1)
         avg, gbs, _ = stage_roofline(ms, n, V, tag)
         legs[tag] = {
             "value": V * K / secs,
@@ -285,30 +339,25 @@ def report_single(qcbflow, args, name, L, beta, seed, sweeps, K, every):
         log(f"  {tag}: {legs[tag]['value']:.4e} site-steps/s, stage {avg:.4f} ms "
             f"({gbs:.0f} GB/s), E(t={tqe[-1, 0]:.2f}) = {tqe[-1, 2]:.6g}, Q = {tqe[-1, 1]:.6g}")
         h.close()
-    copy_ms = None
-    if args.calibrate_copy_mib > 0:
-        copy_ms = src.copy_bench(args.calibrate_copy_mib << 20, 10)
     src.close()
     t64, p64 = series["fp64"]
     t32, p32 = series["fp32_mixed"]
     acc = compare(t64, p64, t32, p32)
-    # the reference's E is (1/2) sum |F|^2 / V with F = (C - C^dag)/8, half the
-    # usual normalisation, so t^2 E_ref = 0.15 is the customary t0 and 0.3 the
-    # reference's literal threshold
     t0s = {}
     for thr in (0.05, 0.1, 0.15, 0.3):
         a, b = t_at(t64, thr), t_at(t32, thr)
         t0s[f"{thr}"] = {"fp64": a, "fp32_mixed": b,
                          "diff": None if a is None or b is None else abs(a - b)}
     acc["t_at_t2E"] = t0s
-    log(f"  fp32-mixed vs fp64 at t={acc['t_gate']:.2f}: dE/E {acc['dE_rel_at_t_gate']:.2e}, "
-        f"dQ {acc['dQ_abs_at_t_gate']:.2e}, dP/P {acc['dP_rel_at_t_gate']:.2e}")
+    if acc["t_gate"] is not None:
+        log(f"  fp32-mixed vs fp64 at t={acc['t_gate']:.2f}: dE/E {acc['dE_rel_at_t_gate']:.2e}, "
+            f"dQ {acc['dQ_abs_at_t_gate']:.2e}, dP/P {acc['dP_rel_at_t_gate']:.2e}")
     return {
-        "report": name,
+        "report": "precision",
         "metric": METRIC,
         "unit": UNIT,
         "config": {"workload": f"Wilson flow, {'x'.join(map(str, L))}, eps={args.eps}, "
-                               f"{K} RK3 steps, measured every {every}",
+                               f"{K} RK3 steps, measured every step",
                    "lattice": list(L), "sites": V, "beta": beta, "seed": seed,
                    "heatbath_sweeps": sweeps},
         "fp64": legs["fp64"],
@@ -316,8 +365,6 @@ def report_single(qcbflow, args, name, L, beta, seed, sweeps, K, every):
         "fp32_mixed_vs_fp64": legs["fp32_mixed"]["value"] / legs["fp64"]["value"],
         "accuracy_fp32_mixed_vs_fp64": acc,
         "data": "synthetic: GPU Haar hot start + GPU heatbath",
-        "copy_calibration": None if copy_ms is None else {"mib": args.calibrate_copy_mib,
-                                                          "avg_ms": copy_ms},
     }
 
 
@@ -409,83 +456,33 @@ def report_ensemble(qcbflow, args):
     out["fp32_mixed_vs_fp64"] = out["fp32_mixed"]["value"] / out["fp64"]["value"]
     per = [compare(res["fp64"][0][c], res["fp64"][1][c], res["fp32_mixed"][0][c],
                    res["fp32_mixed"][1][c]) for c in range(64)]
+    keys = ("dE_rel_at_t_gate", "dQ_abs_at_t_gate", "dP_rel_at_t_gate", "max_dE_rel",
+            "max_dQ_abs", "max_dP_rel")
     out["accuracy_fp32_mixed_vs_fp64"] = {
-        k: max(d[k] for d in per) for k in ("dE_rel_at_t_gate", "dQ_abs_at_t_gate",
-                                            "dP_rel_at_t_gate", "max_dE_rel", "max_dQ_abs",
-                                            "max_dP_rel")}
+        k: (max(d[k] for d in per) if per[0][k] is not None else None) for k in keys}
     out["accuracy_fp32_mixed_vs_fp64"]["t_gate"] = per[0]["t_gate"]
     out["accuracy_fp32_mixed_vs_fp64"]["note"] = "maximum over the 64 configurations"
     return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None,
-                    help="timed RK3 steps (default 200; 250/150/400 for the reports)")
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--L", type=int, nargs="+", default=None)
-    ap.add_argument("--eps", type=float, default=0.02)
-    ap.add_argument("--seed", type=int, default=7)
-    ap.add_argument("--beta", type=float, default=6.0)
-    ap.add_argument("--sweeps", type=int, default=None,
-                    help="heatbath sweeps of the input (default 50)")
-    ap.add_argument("--report", choices=("precision", "ensemble", "large"), default=None)
-    ap.add_argument("--cpu-steps", type=int, default=12)  # about 11 s on 16 host threads
-    ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-mixed leg")
-    ap.add_argument("--map-mode", type=int, default=None,
-                    help="workgroup->tile order (qcbf_tune); default: the library's")
-    ap.add_argument("--map-win", type=int, default=None)
-    ap.add_argument("--tile-shape", type=int, default=None, help="qcbf_tune tile-shape code")
-    ap.add_argument("--threads-per-site", type=int, choices=(1, 2, 4), default=None,
-                    help="threads per site of the RK3 stage kernels (default 4)")
-    ap.add_argument("--autotune", action="store_true",
-                    help="single lattice: run qcbf_autotune before the warmup (default: the "
-                         "library's launch shape, which the autotuner matches within noise)")
-    ap.add_argument("--no-autotune", action="store_true",
-                    help="ensemble report: skip qcbf_batch_autotune")
-    ap.add_argument("--no-fuse", action="store_true",
-                    help="measure with a separate kernel instead of inside the next RK3 stage 1")
-    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
-                    help="also run the plain copy kernel on this many MiB (traffic calibration)")
-    args = ap.parse_args()
-    args.steps_given = args.steps is not None
-
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
-              "the whole lattice lives on a single MI355X.", file=sys.stderr)
-        sys.exit(2)
-
-    import qcbflow
-
-    name, cus = qcbflow.device_info()
-    log(f"device {name}, {cus} CUs")
-    if args.report == "precision":
-        K = args.steps if args.steps_given else 250
-        out = report_single(qcbflow, args, "precision", lattice_of(args, (48,) * 4), 6.2, 11,
-                            args.sweeps if args.sweeps is not None else 100, K, 1)
-        print(json.dumps(out), flush=True)
-        return
-    if args.report == "large":
-        K = args.steps if args.steps_given else 400
-        out = report_single(qcbflow, args, "large", lattice_of(args, (64,) * 4), 6.4, 2026,
-                            args.sweeps if args.sweeps is not None else 50, K, 10)
-        print(json.dumps(out), flush=True)
-        return
-    if args.report == "ensemble":
-        print(json.dumps(report_ensemble(qcbflow, args)), flush=True)
-        return
-
-    L = lattice_of(args, (32,) * 4)
+def headline(qcbflow, args):
+    """The single-lattice workload of the bench line (a preset, by default the
+    64^4 lattice measured every 10 steps): FP64 timed leg (value), FP32-mixed
+    timed leg, accuracy leg to t = 2, flow-only and PCIe-inclusive rates, the
+    copy kernel, and the CPU baseline on the same field."""
+    Lp, beta_p, seed_p, sweeps_p, every_p, K_p = PRESETS[args.preset]
+    L = lattice_of(args, Lp)
     V = volume(L)
-    K = args.steps if args.steps_given else 200
-    sweeps = args.sweeps if args.sweeps is not None else 50
-    log(f"lattice {L}, V = {V}; hot start seed {args.seed}, {sweeps} heatbath sweeps at "
-        f"beta {args.beta}")
+    K = args.steps if args.steps_given else K_p
+    beta = args.beta if args.beta is not None else beta_p
+    seed = args.seed if args.seed is not None else seed_p
+    sweeps = args.sweeps if args.sweeps is not None else sweeps_p
+    every = every_p if K % every_p == 0 else math.gcd(K, every_p)
+    lat = "x".join(map(str, L))
+    log(f"lattice {L}, V = {V}; hot start seed {seed}, {sweeps} heatbath sweeps at beta {beta}; "
+        f"{K} timed RK3 steps, measured every {every}")
     t0 = time.perf_counter()
-    src = thermalised(qcbflow, L, args.beta, args.seed, sweeps)
+    src = thermalised(qcbflow, L, beta, seed, sweeps)
     log(f"thermalised in {time.perf_counter() - t0:.2f} s, plaquette {src.plaquette():.6f}")
 
     gf = qcbflow.GradientFlow(L, args.eps)
@@ -493,8 +490,8 @@ def main():
     shape64 = setup_shape(qcbflow, gf, args)
     log(f"fp64 launch shape (map mode, window, tile code, fused, threads/site): {shape64}")
 
-    # timed region: K RK3 steps, K+1 measurements, stage kernels bracketed by events
-    secs, tqe, plaq, ms, launches = timed_flow(gf, src, K, args.warmup)
+    # timed region: K RK3 steps, K/every + 1 measurements, every launch timed
+    secs, tqe, plaq, ms, launches = timed_flow(gf, src, K, args.warmup, every, args.time_every)
     value = V * K / secs
     log(f"timed: {K} steps in {secs:.4f} s -> {value:.4e} site-steps/s; "
         f"E(t={tqe[-1, 0]:.2f}) = {tqe[-1, 2]:.6g}, Q = {tqe[-1, 1]:.6g}")
@@ -539,7 +536,7 @@ def main():
         gm = qcbflow.GradientFlow(L, args.eps, qcbflow.FP32_MIXED)
         gm.copy_from(src)
         shape32 = setup_shape(qcbflow, gm, args)
-        msecs, mtqe, mplaq, mms, mn = timed_flow(gm, src, K, args.warmup)
+        msecs, mtqe, mplaq, mms, mn = timed_flow(gm, src, K, args.warmup, every, args.time_every)
         mstage, mach, _ = stage_roofline(mms, mn, V, "fp32_mixed")
         mixed = {
             "value": V * K / msecs,
@@ -553,24 +550,33 @@ def main():
             "bytes_per_site_step": MIXED_BYTES_PER_SITE_STEP,
             "fp32_tflops": V * MIXED_FLOPS_PER_SITE_STEP / (3 * mstage * 1e-3) / 1e12,
             "launch_shape": shape32,
-            "accuracy_vs_fp64": compare(tqe, plaq, mtqe, mplaq),
+            "exp_warnings": gm.exp_warnings(),
+            "device_memory": device_memory(V, "fp32_mixed"),
         }
         mixed["fp32_frac"] = mixed["fp32_tflops"] / FP32_PEAK_TFLOPS
         mixed["fp32_frac_unpacked"] = mixed["fp32_tflops"] / FP32_UNPACKED_TFLOPS
         log(f"fp32-mixed: {mixed['value']:.4e} site-steps/s ({mixed['vs_fp64']:.2f}x fp64), "
-            f"stage {mstage:.4f} ms, dE/E at t=2 {mixed['accuracy_vs_fp64']['dE_rel_at_t_gate']:.2e}")
+            f"stage {mstage:.4f} ms")
         gm.close()
+        # the north_star gates at t = 2, on a leg of its own (2/eps steps)
+        acc = accuracy_leg(qcbflow, src, L, args.eps, every)
+        mixed["accuracy_vs_fp64"] = acc
+        log(f"fp32-mixed vs fp64 at t={acc['t_gate']:.2f} ({acc['rk3_steps']} steps): "
+            f"dE/E {acc['dE_rel_at_t_gate']:.2e}, dQ {acc['dQ_abs_at_t_gate']:.2e}, "
+            f"dP/P {acc['dP_rel_at_t_gate']:.2e}")
+    src.close()
 
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
-        links = host_links
-        threads = min(args.cpu_threads, os.cpu_count() or 1)
-        log(f"cpu baseline: {args.cpu_steps} steps on {threads} threads ...")
-        cpu = cpu_baseline(links, L, args.eps, args.cpu_steps, threads)
+        nproc, model = host_cpus()
+        threads = min(args.cpu_threads, nproc)
+        log(f"cpu baseline: {args.cpu_steps} RK3 step(s) + 1 measurement on {threads} of "
+            f"{nproc} CPUs ({model}) ...")
+        cpu = cpu_baseline(host_links, L, args.eps, every, threads, args.cpu_steps)
         log(f"cpu baseline: {cpu['value']:.4e} {UNIT}")
-    src.close()
+    del host_links
 
-    out = {
+    return {
         "metric": METRIC,
         "value": value,
         "unit": UNIT,
@@ -582,17 +588,18 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": f"synthetic: Haar-random SU(3) hot start (seed {args.seed}) thermalised by "
-                f"{sweeps} heatbath sweeps at beta={args.beta}, generated on the GPU",
+        "data": f"synthetic: Haar-random SU(3) hot start (seed {seed}) thermalised by "
+                f"{sweeps} heatbath sweeps at beta={beta}, generated on the GPU",
         "config": {
-            "workload": f"Wilson flow FP64, {'x'.join(map(str, L))} lattice, eps={args.eps}, "
-                        "plaquette+Q+E measured every RK3 step",
+            "workload": f"Wilson flow FP64, {lat} lattice ({V} sites), eps={args.eps}, "
+                        f"plaquette+Q+E measured every {every} RK3 step(s)",
             "lattice": list(L),
             "sites": V,
             "eps": args.eps,
-            "beta": args.beta,
+            "beta": beta,
+            "seed": seed,
             "heatbath_sweeps": sweeps,
-            "measure_every_rk3_steps": 1,
+            "measure_every_rk3_steps": every,
             "precision": "fp64",
             "parallelism": "single GPU",
             "launch_shape": shape64,
@@ -601,7 +608,8 @@ def main():
             "bound": "hbm",
             "kernel": "k_rk3_stage<f64, " + "|".join(str(k + 1) for k in range(3) if launches[k])
                       + "> (one whole RK3 stage per launch"
-                      + ("; stage 1 runs fused with the measurement)" if launches[4] else ")"),
+                      + ("; the stage 1 after a measurement runs fused with it and is "
+                         "reported as stage1_measure)" if launches[4] else ")"),
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -610,7 +618,7 @@ def main():
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": avg_stage_ms,
             "timed_launches": launches[0] + launches[1] + launches[2],
-            "timed_every_nth_launch": TIME_EVERY,
+            "timed_every_nth_launch": args.time_every,
             "fp64_tflops": flops_rate,
             "fp64_frac": flops_rate / FP64_PEAK_TFLOPS,
             "copy_kernel_gbs": copy_gbs,
@@ -622,10 +630,71 @@ def main():
         "flow_only_value": flow_only,
         "pcie_inclusive_value": pcie_value,
         "host_transfer_ms": {"up": t_up * 1e3, "down": t_down * 1e3},
+        "device_memory": device_memory(V, "fp64"),
         "fp32_mixed": mixed,
         "cpu_baseline": cpu,
+        "copy_calibration": {"mib": copy_mib, "avg_ms": copy_ms},
     }
-    out["copy_calibration"] = {"mib": copy_mib, "avg_ms": copy_ms}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed RK3 steps (default: 400 for the 64^4 workload, 200 for c2; "
+                         "250/150 for the reports)")
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--preset", choices=sorted(PRESETS), default="large",
+                    help="single-lattice workload: large = 64^4, beta 6.4, seed 2026, measured "
+                         "every 10 steps (default); c2 = 32^4, beta 6.0, seed 7, every step")
+    ap.add_argument("--L", type=int, nargs="+", default=None)
+    ap.add_argument("--eps", type=float, default=0.02)
+    ap.add_argument("--seed", type=int, default=None)
+    ap.add_argument("--beta", type=float, default=None)
+    ap.add_argument("--sweeps", type=int, default=None,
+                    help="heatbath sweeps of the input (default: the workload's)")
+    ap.add_argument("--report", choices=("precision", "ensemble"), default=None)
+    ap.add_argument("--cpu-steps", type=int, default=1,
+                    help="oracle RK3 steps of the CPU baseline (about 12 s each at 64^4 on 16 "
+                         "host threads)")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-mixed legs")
+    ap.add_argument("--time-every", type=int, default=1,
+                    help="time every n-th launch of each kernel class with HIP events")
+    ap.add_argument("--map-mode", type=int, default=None,
+                    help="workgroup->tile order (qcbf_tune); default: the library's")
+    ap.add_argument("--map-win", type=int, default=None)
+    ap.add_argument("--tile-shape", type=int, default=None, help="qcbf_tune tile-shape code")
+    ap.add_argument("--threads-per-site", type=int, choices=(1, 2, 4), default=None,
+                    help="threads per site of the RK3 stage kernels (default 4)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="single lattice: run qcbf_autotune before the warmup (default: the "
+                         "library's launch shape)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="ensemble report: skip qcbf_batch_autotune")
+    ap.add_argument("--no-fuse", action="store_true",
+                    help="measure with a separate kernel instead of inside the next RK3 stage 1")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="run the plain copy kernel on this many MiB (default 1024)")
+    args = ap.parse_args()
+    args.steps_given = args.steps is not None
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
+              "the whole lattice lives on a single MI355X.", file=sys.stderr)
+        sys.exit(2)
+
+    import qcbflow
+
+    name, cus = qcbflow.device_info()
+    log(f"device {name}, {cus} CUs")
+    if args.report == "precision":
+        out = report_precision(qcbflow, args)
+    elif args.report == "ensemble":
+        out = report_ensemble(qcbflow, args)
+    else:
+        out = headline(qcbflow, args)
     print(json.dumps(out), flush=True)
 
 
