@@ -264,13 +264,15 @@ def setup_shape(qcbflow, h, args):
                      (qcbflow.TUNE_THREADS_PER_SITE, args.threads_per_site)):
         if val is not None:
             h.tune(key, val)
+    if args.lds_stage is not None and h.precision == qcbflow.FP64:
+        h.tune(qcbflow.TUNE_LDS_STAGE, args.lds_stage)
     if args.autotune:
         h.autotune(trial_steps=2)
     if args.no_fuse:
         h.tune(qcbflow.TUNE_FUSED_MEASURE, 0)
     return [h.get_tune(k) for k in (qcbflow.TUNE_MAP_MODE, qcbflow.TUNE_MAP_WIN,
                                     qcbflow.TUNE_TILE_SHAPE, qcbflow.TUNE_FUSED_MEASURE,
-                                    qcbflow.TUNE_THREADS_PER_SITE)]
+                                    qcbflow.TUNE_THREADS_PER_SITE, qcbflow.TUNE_LDS_STAGE)]
 
 
 def lattice_of(args, default):
@@ -488,7 +490,7 @@ def headline(qcbflow, args):
     gf = qcbflow.GradientFlow(L, args.eps)
     gf.copy_from(src)
     shape64 = setup_shape(qcbflow, gf, args)
-    log(f"fp64 launch shape (map mode, window, tile code, fused, threads/site): {shape64}")
+    log(f"fp64 launch shape (map mode, window, tile code, fused, threads/site, lds): {shape64}")
 
     # timed region: K RK3 steps, K/every + 1 measurements, every launch timed
     secs, tqe, plaq, ms, launches = timed_flow(gf, src, K, args.warmup, every, args.time_every)
@@ -606,7 +608,8 @@ def headline(qcbflow, args):
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_rk3_stage<f64, " + "|".join(str(k + 1) for k in range(3) if launches[k])
+            "kernel": ("k_rk3_stage_lds<" if shape64[5] else "k_rk3_stage<f64, ")
+                      + "|".join(str(k + 1) for k in range(3) if launches[k])
                       + "> (one whole RK3 stage per launch"
                       + ("; the stage 1 after a measurement runs fused with it and is "
                          "reported as stage1_measure)" if launches[4] else ")"),
@@ -668,6 +671,8 @@ def main():
     ap.add_argument("--tile-shape", type=int, default=None, help="qcbf_tune tile-shape code")
     ap.add_argument("--threads-per-site", type=int, choices=(1, 2, 4), default=None,
                     help="threads per site of the RK3 stage kernels (default 4)")
+    ap.add_argument("--lds-stage", type=int, choices=(0, 1), default=None,
+                    help="FP64 stage kernel: 1 = shared links staged in LDS, 0 = plain")
     ap.add_argument("--autotune", action="store_true",
                     help="single lattice: run qcbf_autotune before the warmup (default: the "
                          "library's launch shape)")

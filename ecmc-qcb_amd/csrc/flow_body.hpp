@@ -202,35 +202,48 @@ __device__ inline alg8<typename F::real> force_at(const F& f, unsigned s, const 
     return neg_proj_ta(mul(u, st));
 }
 
-template <prec P, int STAGE, int MU, bool XROWS = false, class Hook = no_staple_hook>
-__device__ inline void stage_body(const typename storage<P>::field& f,
-                                  typename storage<P>::vec* __restrict__ Uout,
-                                  typename storage<P>::avec* __restrict__ acc, unsigned s,
-                                  const nbr& n, double cz, double ca, bool& warn,
-                                  Hook&& hook = Hook{}) {
+// The RK3 combination of the force z of U_mu(s) = u with the accumulator,
+// the exponential and the update exp(X) u into Uout (compute_w1/w2/w3 and
+// replace_force_0, gradient_flow.cpp:40-121).
+template <prec P, int STAGE, int MU>
+__device__ inline void stage_update(const alg8<typename storage<P>::real>& z,
+                                    const mat3<typename storage<P>::real>& u, unsigned V,
+                                    typename storage<P>::vec* __restrict__ Uout,
+                                    typename storage<P>::avec* __restrict__ acc, unsigned s,
+                                    double cz, double ca, bool& warn) {
     using S = storage<P>;
     using T = typename S::real;
-    mat3<T> u;
-    const alg8<T> z = force_at<typename S::field, MU, XROWS>(f, s, n, u, hook);
     const T czr = (T)cz, car = (T)ca;
     alg8<T> x;
     if constexpr (STAGE == 1) {
 #pragma unroll
         for (int k = 0; k < 8; k++) x.a[k] = czr * z.a[k];
-        S::store_alg(acc, f.V, MU, s, z);
+        S::store_alg(acc, V, MU, s, z);
     } else {
-        alg8<T> a = S::load_alg(acc, f.V, MU, s);
+        alg8<T> a = S::load_alg(acc, V, MU, s);
 #pragma unroll
         for (int k = 0; k < 8; k++) x.a[k] = czr * z.a[k] - car * a.a[k];
         if constexpr (STAGE == 2) {
 #pragma unroll
             for (int k = 0; k < 8; k++)
                 a.a[k] = (T)(8.0 / 9.0) * z.a[k] - (T)(17.0 / 36.0) * a.a[k];
-            S::store_alg(acc, f.V, MU, s, a);
+            S::store_alg(acc, V, MU, s, a);
         }
     }
     const mat3<T> e = exp_su3<T, double>(x, &warn);
-    S::store(Uout, f.V, MU, s, mul(e, u));
+    S::store(Uout, V, MU, s, mul(e, u));
+}
+
+template <prec P, int STAGE, int MU, bool XROWS = false, class Hook = no_staple_hook>
+__device__ inline void stage_body(const typename storage<P>::field& f,
+                                  typename storage<P>::vec* __restrict__ Uout,
+                                  typename storage<P>::avec* __restrict__ acc, unsigned s,
+                                  const nbr& n, double cz, double ca, bool& warn,
+                                  Hook&& hook = Hook{}) {
+    using T = typename storage<P>::real;
+    mat3<T> u;
+    const alg8<T> z = force_at<typename storage<P>::field, MU, XROWS>(f, s, n, u, hook);
+    stage_update<P, STAGE, MU>(z, u, f.V, Uout, acc, s, cz, ca, warn);
 }
 
 // The site index through an empty asm statement: the compiler must assume it

@@ -35,6 +35,9 @@ struct lattice {
     // Directions per wavefront of the plain stage kernels (1, 2 or 4; threads
     // per site = 4 / dpw): a workgroup of four wavefronts takes dpw tiles.
     int dpw;
+    // FP64 stage kernel with the tile's shared links staged in LDS
+    // (stage_lds.hip) where it applies: 1, or 0 for the plain kernel.
+    int lds_stage;
 };
 
 // Site and coordinates of lane `lane` (0..63) of flow-kernel tile `tile`.
@@ -143,6 +146,7 @@ inline lattice make_lattice(const int dims[4]) {
         g.ntile[d] = 0;
     }
     g.dpw = 1;
+    g.lds_stage = 1;
     // Slab sweep with 16 slabs per t-slice, or as many more (32 ... 512) as
     // keep a slab at most 256 tiles: at 96^4 a 16-slab slab is 864 tiles,
     // whose links at t-1, t and t+1 outgrow the caches, and 216-tile slabs ran

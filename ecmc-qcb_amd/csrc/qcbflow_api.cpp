@@ -481,6 +481,10 @@ static int rk3_once(qcbf_flow* h, double eps, double* measure_out = nullptr) {
             HIPCHK(h, qcb::launch_rk3_stage1_measure(h->prec, h->U[h->cur], h->U[h->cur ^ 1],
                                                      h->acc, h->g, cz[0], h->dwarn, h->partials,
                                                      measure_out, h->stream));
+        else if (h->prec == qcb::prec::f64 && h->g.lds_stage && qcb::stage_lds_applies(h->g))
+            HIPCHK(h, qcb::launch_rk3_stage_lds(stage, h->U[h->cur], h->U[h->cur ^ 1], h->acc,
+                                                h->g, cz[stage - 1], ca[stage - 1], h->dwarn,
+                                                h->stream));
         else
             HIPCHK(h, qcb::launch_rk3_stage(h->prec, stage, h->U[h->cur], h->U[h->cur ^ 1],
                                             h->acc, h->g, cz[stage - 1], ca[stage - 1], h->dwarn,
@@ -734,6 +738,10 @@ extern "C" int qcbf_tune(qcbf_flow* h, int key, long value) {
                 return fail(h, QCBF_ERR_ARG, "threads per site %ld not 1, 2 or 4", value);
             h->g.dpw = 4 / (int)value;
             return QCBF_OK;
+        case QCBF_TUNE_LDS_STAGE:
+            if (value != 0 && value != 1) return fail(h, QCBF_ERR_ARG, "LDS stage %ld not 0 or 1", value);
+            h->g.lds_stage = (int)value;
+            return QCBF_OK;
         case QCBF_TUNE_TILE_SHAPE: {
             if (value == -1) {
                 qcb::set_tile_shape(h->g, nullptr);
@@ -765,6 +773,7 @@ extern "C" int qcbf_get_tune(const qcbf_flow* h, int key, long* value) {
         case QCBF_TUNE_TILE_SHAPE: *value = tile_code(h->g); return QCBF_OK;
         case QCBF_TUNE_FUSED_MEASURE: *value = h->fuse_measure ? 1 : 0; return QCBF_OK;
         case QCBF_TUNE_THREADS_PER_SITE: *value = 4 / h->g.dpw; return QCBF_OK;
+        case QCBF_TUNE_LDS_STAGE: *value = h->g.lds_stage; return QCBF_OK;
         default: return fail(nullptr, QCBF_ERR_ARG, "unknown tuning key %d", key);
     }
 }

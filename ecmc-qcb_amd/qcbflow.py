@@ -34,6 +34,7 @@ K_STAGE1, K_STAGE2, K_STAGE3, K_MEASURE, K_STAGE1_MEASURE = 0, 1, 2, 3, 4
 TUNE_MAP_MODE, TUNE_MAP_WIN, TUNE_BLOCKS_PER_CU, TUNE_TILE_SHAPE = 0, 1, 2, 3
 TUNE_FUSED_MEASURE = 4
 TUNE_THREADS_PER_SITE = 5
+TUNE_LDS_STAGE = 6
 
 
 def tile_code(bx, by, bz, bt):
