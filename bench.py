@@ -264,7 +264,7 @@ def setup_shape(qcbflow, h, args):
                      (qcbflow.TUNE_THREADS_PER_SITE, args.threads_per_site)):
         if val is not None:
             h.tune(key, val)
-    if args.lds_stage is not None and h.precision == qcbflow.FP64:
+    if args.lds_stage is not None:
         h.tune(qcbflow.TUNE_LDS_STAGE, args.lds_stage)
     if args.autotune:
         h.autotune(trial_steps=2)
@@ -672,7 +672,7 @@ def main():
     ap.add_argument("--threads-per-site", type=int, choices=(1, 2, 4), default=None,
                     help="threads per site of the RK3 stage kernels (default 4)")
     ap.add_argument("--lds-stage", type=int, choices=(0, 1), default=None,
-                    help="FP64 stage kernel: 1 = shared links staged in LDS, 0 = plain")
+                    help="stage kernel: 1 = shared links staged in LDS (default), 0 = plain")
     ap.add_argument("--autotune", action="store_true",
                     help="single lattice: run qcbf_autotune before the warmup (default: the "
                          "library's launch shape)")

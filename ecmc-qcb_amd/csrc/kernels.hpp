@@ -28,12 +28,12 @@ __host__ __device__ inline unsigned measure_blocks(const lattice& g) {
 hipError_t launch_rk3_stage(prec p, int stage, const void* Uin, void* Uout, void* acc,
                             const lattice& g, double cz, double ca, int* warn, hipStream_t st);
 
-// The FP64 stage kernel with the links shared by the four direction
-// wavefronts of a tile staged in LDS (stage_lds.hip): the same flow, bit for
-// bit.  Applies to 64-consecutive-site tiles, one thread per link and V a
-// multiple of 64.
-bool stage_lds_applies(const lattice& g);
-hipError_t launch_rk3_stage_lds(int stage, const void* Uin, void* Uout, void* acc,
+// The stage kernel with the links shared by the four direction wavefronts of
+// a tile staged in LDS (stage_lds.hip): the same flow, bit for bit.  Applies
+// to 64-consecutive-site tiles, one thread per link and V a multiple of 64
+// (FP32-mixed: also whole x-rows per wavefront).
+bool stage_lds_applies(prec p, const lattice& g);
+hipError_t launch_rk3_stage_lds(prec p, int stage, const void* Uin, void* Uout, void* acc,
                                 const lattice& g, double cz, double ca, int* warn,
                                 hipStream_t st);
 

@@ -481,8 +481,8 @@ static int rk3_once(qcbf_flow* h, double eps, double* measure_out = nullptr) {
             HIPCHK(h, qcb::launch_rk3_stage1_measure(h->prec, h->U[h->cur], h->U[h->cur ^ 1],
                                                      h->acc, h->g, cz[0], h->dwarn, h->partials,
                                                      measure_out, h->stream));
-        else if (h->prec == qcb::prec::f64 && h->g.lds_stage && qcb::stage_lds_applies(h->g))
-            HIPCHK(h, qcb::launch_rk3_stage_lds(stage, h->U[h->cur], h->U[h->cur ^ 1], h->acc,
+        else if (h->g.lds_stage && qcb::stage_lds_applies(h->prec, h->g))
+            HIPCHK(h, qcb::launch_rk3_stage_lds(h->prec, stage, h->U[h->cur], h->U[h->cur ^ 1], h->acc,
                                                 h->g, cz[stage - 1], ca[stage - 1], h->dwarn,
                                                 h->stream));
         else

@@ -1,9 +1,11 @@
-"""The FP64 RK3 stage kernel with the tile's shared links staged in LDS
-(QCBF_TUNE_LDS_STAGE = 1) against the plain stage kernel: the same products
-on the same words in the same order, so fields and measured series must be
-bit-identical.  Lattices with whole x-rows per wavefront (x-neighbours read
-from LDS) and without (Lx not dividing 64), and one whose volume is not a
-multiple of 64 (the plain kernel runs)."""
+"""The RK3 stage kernel with the tile's shared links staged in LDS
+(QCBF_TUNE_LDS_STAGE = 1) against the plain stage kernel, in both storage
+forms: the same products on the same words in the same order, so fields and
+measured series must be bit-identical.  Lattices with whole x-rows per
+wavefront (x-neighbours read from LDS) and without (Lx not dividing 64: the
+FP64 form still stages the same-site links, the FP32-mixed form runs the plain
+kernel), and one whose volume is not a multiple of 64 (the plain kernel
+runs)."""
 import numpy as np
 import pytest
 
@@ -17,14 +19,15 @@ def say(msg):
     print(msg, flush=True)
 
 
+@pytest.mark.parametrize("prec", [qcbflow.FP64, qcbflow.FP32_MIXED])
 @pytest.mark.parametrize("L", [(8, 8, 8, 8), (16, 8, 4, 8), (32, 4, 4, 4), (64, 2, 2, 4),
                                (12, 8, 8, 8), (6, 4, 8, 4), (3, 5, 4, 7)])
-def test_lds_stage_bit_identical(L):
-    say(f"LDS stage vs plain stage, lattice {L}")
+def test_lds_stage_bit_identical(L, prec):
+    say(f"LDS stage vs plain stage, lattice {L}, precision {prec}")
     f = oracle.hot_start(L, 99)
     out = {}
     for lds in (0, 1):
-        with qcbflow.GradientFlow(L, 0.02) as gf:
+        with qcbflow.GradientFlow(L, 0.02, prec) as gf:
             gf.tune(qcbflow.TUNE_LDS_STAGE, lds)
             assert gf.get_tune(qcbflow.TUNE_LDS_STAGE) == lds
             gf.copy(f)
