@@ -8,6 +8,7 @@
 #pragma once
 #include <complex>
 #include <cstddef>
+#include <random>
 #include <vector>
 
 #include "../geometry/GeometryCB.h"
@@ -46,4 +47,8 @@ public:
     // Periodic halos from the interior (what exchange_halos_cascade yields on
     // a single rank).
     void fill_halos(const GeometryCB& geo);
+    // Haar-random links (GaugeField::hot_start, src/gauge/GaugeField.cpp:10-20),
+    // generated on the device from a counter generator keyed by one draw of
+    // rng; halos filled.
+    void hot_start(const GeometryCB& geo, std::mt19937_64& rng);
 };

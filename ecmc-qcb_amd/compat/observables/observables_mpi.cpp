@@ -3,30 +3,12 @@
 #include <iostream>
 #include <memory>
 
+#include "../device_scratch.h"
 #include "../io/io.h"
 
 namespace {
-// A device lattice kept between calls for observables of host fields.
-struct scratch_lattice {
-    int L = 0;
-    qcbf_flow* h = nullptr;
-    std::vector<double> buf;
-    ~scratch_lattice() { qcbf_destroy(h); }
-};
-
 qcbf_flow* upload(const GaugeField& field, const GeometryCB& geo) {
-    static scratch_lattice s;
-    if (s.L != geo.L_int) {
-        qcbf_destroy(s.h);
-        s.h = nullptr;
-        const int dims[4] = {geo.L_int, geo.L_int, geo.L_int, geo.L_int};
-        qcbf_compat_check(qcbf_create(dims, QCBF_FP64, &s.h), nullptr, "qcbf_create");
-        s.L = geo.L_int;
-        s.buf.resize(geo.V_int * 72);
-    }
-    field.pack_interior(geo, s.buf.data());
-    qcbf_compat_check(qcbf_set_links(s.h, s.buf.data()), s.h, "qcbf_set_links");
-    return s.h;
+    return qcbcompat::upload(field, geo);
 }
 }  // namespace
 
@@ -44,6 +26,40 @@ double mpi::observables::mean_plaquette_global(GaugeField& field, const Geometry
     qcbf_compat_check(qcbf_plaquette(h, &p), h, "qcbf_plaquette");
     return p;
 }
+
+int mpi::observables::levi_civita(int mu, int nu, int rho, int sigma) {
+    const int p[4] = {mu, nu, rho, sigma};
+    int sign = 1;
+    for (int i = 0; i < 4; i++)
+        for (int j = i + 1; j < 4; j++) {
+            if (p[i] == p[j]) return 0;
+            if (p[i] > p[j]) sign = -sign;
+        }
+    return sign;
+}
+
+SU3 mpi::observables::clover_site(const GaugeField& field, const GeometryCB& geo, size_t site,
+                                  int mu, int nu) {
+    if (mu == nu) std::cerr << "mu = nu => G(site, mu, nu) = 0\n";  // observables_mpi.cpp:68
+    qcbf_flow* h = upload(field, geo);
+    double out[18];
+    qcbf_compat_check(qcbf_clover_site(h, qcbcompat::interior_site(geo, site), mu, nu, out), h,
+                      "qcbf_clover_site");
+    SU3 F;
+    for (int k = 0; k < 9; k++) F.data()[k] = Complex(out[2 * k], out[2 * k + 1]);
+    return F;
+}
+
+std::pair<double, double> mpi::observables::local_q_e_clover(const GaugeField& field,
+                                                             const GeometryCB& geo, size_t site) {
+    qcbf_flow* h = upload(field, geo);
+    double q = 0.0, e = 0.0;
+    qcbf_compat_check(qcbf_local_q_e(h, qcbcompat::interior_site(geo, site), &q, &e), h,
+                      "qcbf_local_q_e");
+    return {q, e};
+}
+
+void mpi::observables::release_device_scratch() { qcbcompat::release_device_scratch(); }
 
 std::pair<double, double> mpi::observables::topo_q_e_clover(const GaugeField& field,
                                                             const GeometryCB& geo) {

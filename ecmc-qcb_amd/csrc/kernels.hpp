@@ -69,6 +69,13 @@ hipError_t launch_measure(prec p, const void* U, const lattice& g, double* parti
 // tiles -> out[0..2]).
 hipError_t launch_reduce_measure(const double* partials, const lattice& g, double* out,
                                  hipStream_t st);
+// clover_site (observables_mpi.cpp:66-90) at one site: out[0..17] = F_munu(x)
+// = (C - C^dag)/8, row-major (re, im); zero for mu == nu.
+hipError_t launch_clover_site(prec p, const void* U, const lattice& g, unsigned site, int mu,
+                              int nu, double* out, hipStream_t st);
+// local_q_e_clover (observables_mpi.cpp:93-112) at one site: out = {q(x), e(x)}.
+hipError_t launch_local_q_e(prec p, const void* U, const lattice& g, unsigned site, double* out,
+                            hipStream_t st);
 // max over links of |U U^dag - 1|_max and of |det U - 1|, into out[0..1]
 // (partials sized 2*measure_blocks)
 hipError_t launch_unitarity(prec p, const void* U, const lattice& g, double* partials,
@@ -82,8 +89,10 @@ hipError_t launch_aos_to_field(prec p, const void* aos, int kind, void* U, const
                                unsigned site0, unsigned nsites, hipStream_t st);
 hipError_t launch_field_to_aos(prec p, const void* U, double* aos, const lattice& g,
                                unsigned site0, unsigned nsites, bool bswap, hipStream_t st);
+// nsites su(3) elements per direction from site0 on, as full 3x3 matrices in
+// the link order (72 doubles per site).
 hipError_t launch_alg_to_aos(prec p, const void* alg, double* aos, const lattice& g,
-                             hipStream_t st);
+                             unsigned site0, unsigned nsites, hipStream_t st);
 hipError_t launch_hot_start(prec p, void* U, const lattice& g, uint64_t seed, hipStream_t st);
 hipError_t launch_cold_start(prec p, void* U, const lattice& g, hipStream_t st);
 // out_mu(x) = in_mu(x - shift) on the periodic lattice, 0 <= shift[d] < L[d].

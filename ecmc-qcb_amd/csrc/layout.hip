@@ -147,27 +147,31 @@ hipError_t launch_field_to_aos(prec p, const void* U, double* aos, const lattice
 // su(3) field (8 reals per link) -> full 3x3 AoS doubles
 template <prec P>
 __global__ __launch_bounds__(256) void k_alg_to_aos(const typename storage<P>::avec* __restrict__ alg,
-                                                    double* __restrict__ aos, unsigned V) {
-    const unsigned s = blockIdx.x * kSitesPerBlock + threadIdx.x;
-    if (s >= V) return;
+                                                    double* __restrict__ aos, unsigned V,
+                                                    unsigned site0, unsigned nsites) {
+    const unsigned i = blockIdx.x * kSitesPerBlock + threadIdx.x;
+    if (i >= nsites) return;
+    const unsigned s = site0 + i;
     const int mu = threadIdx.y;
     const auto m = alg_to_mat(storage<P>::load_alg(alg, V, mu, s));
-    double* d = aos + ((size_t)s * 4 + mu) * 18;
+    double* d = aos + ((size_t)i * 4 + mu) * 18;
     for (int k = 0; k < 9; k++) {
         d[2 * k] = (double)m.e[k / 3][k % 3].r;
         d[2 * k + 1] = (double)m.e[k / 3][k % 3].i;
     }
 }
 hipError_t launch_alg_to_aos(prec p, const void* alg, double* aos, const lattice& g,
-                             hipStream_t st) {
+                             unsigned site0, unsigned nsites, hipStream_t st) {
     (void)hipGetLastError();  // drop a stale error of an earlier call
-    const unsigned nb = (g.V + kSitesPerBlock - 1) / kSitesPerBlock;
+    if (nsites == 0) return hipSuccess;
+    if (site0 > g.V || nsites > g.V - site0) return hipErrorInvalidValue;
+    const unsigned nb = (nsites + kSitesPerBlock - 1) / kSitesPerBlock;
     if (p == prec::f64)
         k_alg_to_aos<prec::f64><<<nb, dim3(kSitesPerBlock, 4), 0, st>>>(
-            static_cast<const double2*>(alg), aos, g.V);
+            static_cast<const double2*>(alg), aos, g.V, site0, nsites);
     else
         k_alg_to_aos<prec::f32m><<<nb, dim3(kSitesPerBlock, 4), 0, st>>>(
-            static_cast<const float4*>(alg), aos, g.V);
+            static_cast<const float4*>(alg), aos, g.V, site0, nsites);
     return hipGetLastError();
 }
 

@@ -134,6 +134,128 @@ hipError_t launch_measure(prec p, const void* U, const lattice& g, double* parti
 }
 
 // ---- unitarity check --------------------------------------------------------
+// ---- one site -----------------------------------------------------------------
+// The link U_d(s) in FP64 whatever the storage form.
+template <class F>
+__device__ inline mat3<double> link64(const F& f, int d, unsigned s) {
+    const auto m = f.link(d, s);
+    mat3<double> r;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) r.e[i][j] = {(double)m.e[i][j].r, (double)m.e[i][j].i};
+    return r;
+}
+template <typename T>
+__device__ inline mat3<double> adj(const mat3<T>& a) {
+    mat3<double> r;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) r.e[i][j] = conj(a.e[j][i]);
+    return r;
+}
+
+// F_munu(x) = (C - C^dag)/8 of clover_site (observables_mpi.cpp:66-90): the four
+// leaves in the reference's order and left-to-right association, each
+// starting and ending at x, counter-clockwise in the (mu, nu) plane; zero for
+// mu == nu (where the reference warns).
+template <class F>
+__device__ inline mat3<double> clover_at(const F& f, const lattice& g, unsigned x, int mu, int nu) {
+    mat3<double> Fm = zero3<double>();
+    if (mu == nu) return Fm;
+    const nbr nx = neighbours(g, x);
+    const unsigned xpmu = x + nx.up[mu], xpnu = x + nx.up[nu];
+    const unsigned xmmu = x + nx.dn[mu], xmnu = x + nx.dn[nu];
+    const nbr nm = neighbours(g, xmmu), np = neighbours(g, xpmu);
+    const unsigned xmmupnu = xmmu + nm.up[nu], xmmumnu = xmmu + nm.dn[nu];
+    const unsigned xpmumnu = xpmu + np.dn[nu];
+    mat3<double> c = mul(mul(mul(link64(f, mu, x), link64(f, nu, xpmu)), adj(link64(f, mu, xpnu))),
+                         adj(link64(f, nu, x)));
+    add_to(c, mul(mul(mul(link64(f, nu, x), adj(link64(f, mu, xmmupnu))), adj(link64(f, nu, xmmu))),
+                  link64(f, mu, xmmu)));
+    add_to(c, mul(mul(mul(adj(link64(f, mu, xmmu)), adj(link64(f, nu, xmmumnu))),
+                      link64(f, mu, xmmumnu)),
+                  link64(f, nu, xmnu)));
+    add_to(c, mul(mul(mul(adj(link64(f, nu, xmnu)), link64(f, mu, xmnu)), link64(f, nu, xpmumnu)),
+                  adj(link64(f, mu, x))));
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) Fm.e[i][j] = 0.125 * (c.e[i][j] - conj(c.e[j][i]));
+    return Fm;
+}
+
+// Re tr(A B)
+__device__ inline double re_tr_mul(const mat3<double>& A, const mat3<double>& B) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) s += A.e[i][k].r * B.e[k][i].r - A.e[i][k].i * B.e[k][i].i;
+    return s;
+}
+__device__ inline double sq_norm(const mat3<double>& A) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) s += norm2(A.e[i][j]);
+    return s;
+}
+
+// One thread: out[0..17] = F_munu at site x (row-major, re/im).
+template <prec P>
+__global__ void k_clover_site(const typename storage<P>::vec* __restrict__ U, lattice g,
+                              unsigned x, int mu, int nu, double* __restrict__ out) {
+    if (threadIdx.x != 0) return;
+    const mat3<double> F = clover_at(storage<P>::view(U, g.V), g, x, mu, nu);
+    for (int k = 0; k < 9; k++) {
+        out[2 * k] = F.e[k / 3][k % 3].r;
+        out[2 * k + 1] = F.e[k / 3][k % 3].i;
+    }
+}
+
+// One thread: local_q_e_clover (observables_mpi.cpp:93-112) at site x,
+// out = {q(x) = (Re tr F01F23 - Re tr F02F13 + Re tr F03F12) / 4pi^2,
+//        e(x) = 1/2 sum_{mu<nu} |F_munu|^2}.
+template <prec P>
+__global__ void k_local_q_e(const typename storage<P>::vec* __restrict__ U, lattice g,
+                            unsigned x, double* __restrict__ out) {
+    if (threadIdx.x != 0) return;
+    const auto f = storage<P>::view(U, g.V);
+    const mat3<double> F01 = clover_at(f, g, x, 0, 1), F02 = clover_at(f, g, x, 0, 2);
+    const mat3<double> F03 = clover_at(f, g, x, 0, 3), F12 = clover_at(f, g, x, 1, 2);
+    const mat3<double> F13 = clover_at(f, g, x, 1, 3), F23 = clover_at(f, g, x, 2, 3);
+    const double q = re_tr_mul(F01, F23) - re_tr_mul(F02, F13) + re_tr_mul(F03, F12);
+    const double e = 0.5 * (sq_norm(F01) + sq_norm(F02) + sq_norm(F03) + sq_norm(F12) +
+                            sq_norm(F13) + sq_norm(F23));
+    out[0] = q * (1.0 / (4.0 * M_PI * M_PI));
+    out[1] = e;
+}
+
+hipError_t launch_clover_site(prec p, const void* U, const lattice& g, unsigned site, int mu,
+                              int nu, double* out, hipStream_t st) {
+    (void)hipGetLastError();  // drop a stale error of an earlier call
+    if (site >= g.V || mu < 0 || mu > 3 || nu < 0 || nu > 3) return hipErrorInvalidValue;
+    if (p == prec::f64)
+        k_clover_site<prec::f64><<<1, 64, 0, st>>>(static_cast<const double2*>(U), g, site, mu, nu, out);
+    else
+        k_clover_site<prec::f32m><<<1, 64, 0, st>>>(static_cast<const float4*>(U), g, site, mu, nu, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_local_q_e(prec p, const void* U, const lattice& g, unsigned site, double* out,
+                            hipStream_t st) {
+    (void)hipGetLastError();  // drop a stale error of an earlier call
+    if (site >= g.V) return hipErrorInvalidValue;
+    if (p == prec::f64)
+        k_local_q_e<prec::f64><<<1, 64, 0, st>>>(static_cast<const double2*>(U), g, site, out);
+    else
+        k_local_q_e<prec::f32m><<<1, 64, 0, st>>>(static_cast<const float4*>(U), g, site, out);
+    return hipGetLastError();
+}
+
 template <prec P>
 __global__ __launch_bounds__(256) void k_unitarity(const typename storage<P>::vec* __restrict__ U,
                                                    lattice g, double* __restrict__ partials) {

@@ -551,15 +551,17 @@ extern "C" int qcbf_get_force(qcbf_flow* h, int i, double* out) {
     CHECK_H(h);
     if ((i != 0 && i != 1) || !out) return fail(h, QCBF_ERR_ARG, "bad force index or null output");
     if (!h->F[i]) return fail(h, QCBF_ERR_STATE, "force_%d has not been computed", i);
-    double* tmp = nullptr;
-    HIPCHK(h, hipMalloc(&tmp, (size_t)h->g.V * 72 * sizeof(double)));
-    hipError_t e = qcb::launch_alg_to_aos(h->prec, h->F[i], tmp, h->g, h->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(out, tmp, (size_t)h->g.V * 72 * sizeof(double), hipMemcpyDeviceToHost,
-                           h->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    (void)hipFree(tmp);
-    if (e != hipSuccess) return fail(h, QCBF_ERR_HIP, "get_force: %s", hipGetErrorString(e));
+    int rc = ensure_stage(h);
+    if (rc) return rc;
+    // through the staging chunk, kChunkSites sites at a time, as qcbf_get_links
+    double* st = static_cast<double*>(h->stage);
+    for (unsigned s0 = 0; s0 < h->g.V; s0 += kChunkSites) {
+        const unsigned n = std::min(kChunkSites, h->g.V - s0);
+        HIPCHK(h, qcb::launch_alg_to_aos(h->prec, h->F[i], st, h->g, s0, n, h->stream));
+        HIPCHK(h, hipMemcpyAsync(out + (size_t)s0 * 72, st, (size_t)n * 72 * sizeof(double),
+                                 hipMemcpyDeviceToHost, h->stream));
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
     return QCBF_OK;
 }
 
@@ -610,6 +612,30 @@ extern "C" int qcbf_measure(qcbf_flow* h, double* plaq, double* E, double* Q) {
     *plaq = r[0];
     *E = r[1];
     *Q = r[2];
+    return QCBF_OK;
+}
+
+extern "C" int qcbf_clover_site(qcbf_flow* h, long site, int mu, int nu, double* out) {
+    CHECK_H(h);
+    if (!out || site < 0 || (unsigned long)site >= h->g.V || mu < 0 || mu > 3 || nu < 0 || nu > 3)
+        return fail(h, QCBF_ERR_ARG, "clover_site: bad site, direction or null output");
+    HIPCHK(h, qcb::launch_clover_site(h->prec, h->U[h->cur], h->g, (unsigned)site, mu, nu, h->dres,
+                                      h->stream));
+    HIPCHK(h, hipMemcpyAsync(out, h->dres, 18 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return QCBF_OK;
+}
+
+extern "C" int qcbf_local_q_e(qcbf_flow* h, long site, double* q, double* e) {
+    CHECK_H(h);
+    if (!q || !e || site < 0 || (unsigned long)site >= h->g.V)
+        return fail(h, QCBF_ERR_ARG, "local_q_e: bad site or null output");
+    HIPCHK(h, qcb::launch_local_q_e(h->prec, h->U[h->cur], h->g, (unsigned)site, h->dres, h->stream));
+    double r[2];
+    HIPCHK(h, hipMemcpyAsync(r, h->dres, sizeof(r), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    *q = r[0];
+    *e = r[1];
     return QCBF_OK;
 }
 

@@ -75,6 +75,8 @@ SIGNATURES = {
     "qcbf_plaquette": (ctypes.c_int, [_vp, _dp]),
     "qcbf_q_e_clover": (ctypes.c_int, [_vp, _dp, _dp]),
     "qcbf_measure": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
+    "qcbf_clover_site": (ctypes.c_int, [_vp, ctypes.c_long, ctypes.c_int, ctypes.c_int, _dp]),
+    "qcbf_local_q_e": (ctypes.c_int, [_vp, ctypes.c_long, _dp, _dp]),
     "qcbf_flow_measure": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp,
                                          _dp]),
     "qcbf_exp_warnings": (ctypes.c_int, [_vp, _ip]),
@@ -287,6 +289,18 @@ class GradientFlow:
         p, e, q = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         self._c(load().qcbf_measure(self._h, ctypes.byref(p), ctypes.byref(e), ctypes.byref(q)))
         return p.value, e.value, q.value
+
+    def clover_site(self, site, mu, nu):
+        """clover_site: F_munu at one site as a (3, 3) complex array."""
+        out = np.empty(18)
+        self._c(load().qcbf_clover_site(self._h, int(site), int(mu), int(nu), _p(out)))
+        return out[0::2].reshape(3, 3) + 1j * out[1::2].reshape(3, 3)
+
+    def local_q_e(self, site):
+        """local_q_e_clover: (q(x), e(x)) at one site."""
+        q, e = ctypes.c_double(), ctypes.c_double()
+        self._c(load().qcbf_local_q_e(self._h, int(site), ctypes.byref(q), ctypes.byref(e)))
+        return q.value, e.value
 
     def flow_measure(self, n_steps_gf, n_rk_steps, eps=None):
         """topo_charge_flowed cadence: returns (tQE of shape (n, 3), plaquette (n,))."""

@@ -154,6 +154,16 @@ int qcbf_q_e_clover(qcbf_flow* h, double* Q, double* E);
 /* Both in one fused pass. */
 int qcbf_measure(qcbf_flow* h, double* plaq, double* E, double* Q);
 
+/* clover_site (src/observables/observables_mpi.cpp:66-90) of the flowing field
+ * at one site (site = ((t*Lz + z)*Ly + y)*Lx + x, 0 <= site < V):
+ * out[18] = F_munu = (C - C^dag)/8 row-major (re, im), the trace kept; zero
+ * for mu == nu (the reference prints a warning there). */
+int qcbf_clover_site(qcbf_flow* h, long site, int mu, int nu, double* out);
+/* local_q_e_clover (src/observables/observables_mpi.cpp:93-112) at one site:
+ * q = (Re tr F01F23 - Re tr F02F13 + Re tr F03F12)/(4 pi^2),
+ * e = 1/2 sum_{mu<nu} |F_munu|_F^2. */
+int qcbf_local_q_e(qcbf_flow* h, long site, double* q, double* e);
+
 /* topo_charge_flowed (src/observables/observables_mpi.cpp:159-186) on the
  * current flowing field: for n = 0..n_steps_gf-1 measure (t, Q, E) at t
  * (tQE[3n..3n+2]) and the plaquette (plaq[n], may be NULL); unless n is the

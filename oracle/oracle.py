@@ -43,6 +43,7 @@ def lib():
         L.orc_plaquette.restype = ctypes.c_double
         L.orc_clover.argtypes = [_dp, _ip, ctypes.c_long, ctypes.c_int, ctypes.c_int, _dp]
         L.orc_q_e_clover.argtypes = [_dp, _ip, _dp, _dp]
+        L.orc_local_q_e.argtypes = [_dp, _ip, ctypes.c_long, _dp]
         L.orc_flow_measure.argtypes = [_dp, _ip, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                        _dp, _dp, _dp]
         L.orc_time_flow.argtypes = [_dp, _ip, ctypes.c_double, ctypes.c_int, ctypes.c_int]
@@ -172,6 +173,13 @@ def clover(f, L, site, mu, nu):
     out = np.empty((3, 3, 2))
     lib().orc_clover(_p(f), _dims(L), int(site), int(mu), int(nu), _p(out))
     return to_complex(out)
+
+
+def local_q_e(f, L, site):
+    """local_q_e_clover at one site: (q(x), e(x))."""
+    out = np.empty(2)
+    lib().orc_local_q_e(_p(f), _dims(L), int(site), _p(out))
+    return float(out[0]), float(out[1])
 
 
 def q_e_clover(f, L):

@@ -337,6 +337,20 @@ static mat3 clover(const double* f, const geom* g, long x, int mu, int nu) {
  * src/observables/observables_mpi.cpp:93-156.
  * q = [Re tr F01F23 - Re tr F02F13 + Re tr F03F12] / (4 pi^2);
  * e = 1/2 sum_{mu<nu} |F_munu|^2;  returns {sum q, sum e / V}. */
+/* local_q_e_clover (observables_mpi.cpp:93-112) at site s: q(s) with its
+ * 1/(4 pi^2) and e(s). */
+static void local_q_e(const double* f, const geom* g, long s, double* q_out, double* e_out) {
+    mat3 F01 = clover(f, g, s, 0, 1), F02 = clover(f, g, s, 0, 2);
+    mat3 F03 = clover(f, g, s, 0, 3), F12 = clover(f, g, s, 1, 2);
+    mat3 F13 = clover(f, g, s, 1, 3), F23 = clover(f, g, s, 2, 3);
+    double q = creal(mtrace(mmul(F01, F23))) - creal(mtrace(mmul(F02, F13))) +
+               creal(mtrace(mmul(F03, F12)));
+    double e = 0.5 * (mnorm2(F01) + mnorm2(F02) + mnorm2(F03) + mnorm2(F12) + mnorm2(F13) +
+                      mnorm2(F23));
+    *q_out = q * (1.0 / (4.0 * M_PI * M_PI));
+    *e_out = e;
+}
+
 static void q_e_clover(const double* f, const geom* g, double* Q, double* E) {
     long nrow = (long)g->L[3] * g->L[2];
     long rowlen = (long)g->L[1] * g->L[0];
@@ -346,15 +360,9 @@ static void q_e_clover(const double* f, const geom* g, double* Q, double* E) {
     for (long r = 0; r < nrow; r++) {
         double sq = 0.0, se = 0.0;
         for (long k = 0; k < rowlen; k++) {
-            long s = r * rowlen + k;
-            mat3 F01 = clover(f, g, s, 0, 1), F02 = clover(f, g, s, 0, 2);
-            mat3 F03 = clover(f, g, s, 0, 3), F12 = clover(f, g, s, 1, 2);
-            mat3 F13 = clover(f, g, s, 1, 3), F23 = clover(f, g, s, 2, 3);
-            double q = creal(mtrace(mmul(F01, F23))) - creal(mtrace(mmul(F02, F13))) +
-                       creal(mtrace(mmul(F03, F12)));
-            double e = 0.5 * (mnorm2(F01) + mnorm2(F02) + mnorm2(F03) + mnorm2(F12) + mnorm2(F13) +
-                              mnorm2(F23));
-            sq += q * (1.0 / (4.0 * M_PI * M_PI));
+            double q, e;
+            local_q_e(f, g, r * rowlen + k, &q, &e);
+            sq += q;
             se += e;
         }
         pq[r] = sq;
@@ -595,6 +603,12 @@ void orc_clover(const double* f, const int L[4], long site, int mu, int nu, doub
     geom g;
     geom_init(&g, L);
     store(out, clover(f, &g, site, mu, nu));
+    geom_free(&g);
+}
+void orc_local_q_e(const double* f, const int L[4], long site, double* out) {
+    geom g;
+    geom_init(&g, L);
+    local_q_e(f, &g, site, &out[0], &out[1]);
     geom_free(&g);
 }
 void orc_q_e_clover(const double* f, const int L[4], double* Q, double* E) {

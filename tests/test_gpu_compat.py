@@ -90,3 +90,92 @@ def test_compat_stage_members_match_rk3_step_and_oracle(golden_dir, tmp_path):
     assert [(r[0], r[1], r[2]) for r in recs] == [("ildg-format", True, True),
                                                  ("ildg-binary-data", False, True)]
     assert np.array_equal(limefile.read_ildg(str(tmp_path / "flowed" / "flowed")), rk3)
+
+
+SAMPLING = os.path.join(ROOT, "ecmc-qcb_amd", "compat", "hb_sampling")
+
+
+def test_sampling_loop_caller_matches_oracle_on_evolving_configurations(tmp_path):
+    """The sampling loop of the reference's heatbath driver
+    (src/bin/gauge_hb_cb.cpp:150-197) through the drop-in: random shift and
+    mpi::heatbathcb::samples on the device, plaquette, and topo_charge_flowed
+    through ONE GradientFlow object on the evolving configuration.  Every
+    measured series equals the oracle's flow of the configuration the caller
+    saved just before measuring (P and E to 1e-12 relative, Q to 1e-10), and
+    the measured configurations differ from one another."""
+    import limefile
+    import oracle
+
+    L = (8, 8, 8, 8)
+    print("compat hb_sampling: 8^4, 3 thermalisation + 6 sampling shifts", flush=True)
+    args = [SAMPLING, str(tmp_path), "run", "8", "6.0", "1234", "3", "6", "2", "1", "1", "2",
+            "11", "2"]
+    out = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr[-2000:]
+    rows = [l.split()[1:] for l in out.stdout.splitlines() if l.startswith("tQE ")]
+    got = {}
+    for k, t, q, e in rows:
+        got.setdefault(int(k), []).append((float(t), float(q), float(e)))
+    assert sorted(got) == [0, 1, 2]
+    fields_seen = []
+    for k in range(3):
+        tqe = np.array(got[k])
+        f = limefile.read_ildg(str(tmp_path / f"run_m{k}" / f"run_m{k}")).reshape(-1, 4, 3, 3, 2)
+        fields_seen.append(f)
+        ref, _, _ = oracle.flow_measure(f.copy(), L, 0.02, 11, 2)
+        assert tqe.shape == (11, 3)
+        assert np.array_equal(tqe[:, 0], ref[:, 0])
+        assert np.max(np.abs(tqe[:, 2] - ref[:, 2]) / ref[:, 2]) < 1e-12
+        assert np.max(np.abs(tqe[:, 1] - ref[:, 1])) < 1e-10
+    assert not np.array_equal(fields_seen[0], fields_seen[1])
+    assert not np.array_equal(fields_seen[1], fields_seen[2])
+    plaq = [float(l.split()[2]) for l in out.stdout.splitlines() if l.startswith("P ")]
+    assert len(plaq) == 6 and all(0.5 < p < 0.7 for p in plaq)
+    topo = (tmp_path / "run" / "run_topo.txt").read_text().splitlines()
+    assert len(topo) == 33
+
+
+def test_compat_observables_per_site_and_levi_civita(tmp_path):
+    """clover_site, local_q_e_clover and levi_civita of the drop-in
+    (src/observables/observables_mpi.h:8-19) against the oracle, through a
+    C++ caller that only calls observables (no GradientFlow), and exits
+    cleanly."""
+    import limefile
+    import oracle
+
+    src = os.path.join(ROOT, "tests", "compat_observables_check.cpp")
+    exe = str(tmp_path / "obs_check")
+    compat = os.path.join(ROOT, "ecmc-qcb_amd", "compat")
+    pkg = os.path.join(ROOT, "ecmc-qcb_amd")
+    subprocess.run(["g++", "-O1", "-std=c++20", "-I", compat, "-I", os.path.join(ROOT, "include"),
+                    src, os.path.join(compat, "libqcbcompat.a"), "-L", pkg, "-lqcbflow",
+                    f"-Wl,-rpath,{pkg}", "-o", exe], check=True, timeout=120)
+    d = tmp_path / "conf"
+    d.mkdir()
+    L = (6, 6, 6, 6)
+    f = oracle.rk3(oracle.hot_start(L, 77), L, 0.02, 2)
+    limefile.write_ildg(str(d / "conf"), f, L)
+    out = subprocess.run([exe, str(tmp_path), "conf", "6"], capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stderr[-2000:]
+    lines = out.stdout.splitlines()
+    for l in lines:
+        w = l.split()
+        if w[0] == "F":  # F <x> <y> <z> <t> <mu> <nu> 18 doubles (extended-box coordinates)
+            x, y, z, t, mu, nu = map(int, w[1:7])
+            site = (((t - 1) % 6 * 6 + (z - 1) % 6) * 6 + (y - 1) % 6) * 6 + (x - 1) % 6
+            v = np.array(w[7:], dtype=float)
+            got = v[0::2].reshape(3, 3) + 1j * v[1::2].reshape(3, 3)
+            ref = oracle.clover(f, L, site, mu, nu) if mu != nu else np.zeros((3, 3))
+            assert np.abs(got - ref).max() < 1e-14
+        elif w[0] == "QE":
+            x, y, z, t = map(int, w[1:5])
+            site = (((t - 1) % 6 * 6 + (z - 1) % 6) * 6 + (y - 1) % 6) * 6 + (x - 1) % 6
+            q, e = oracle.local_q_e(f, L, site)
+            assert abs(float(w[5]) - q) < 1e-15 and abs(float(w[6]) - e) < 1e-13 * e
+        elif w[0] == "EPS":
+            assert w[1] == "ok"
+    assert sum(1 for l in lines if l.startswith("F ")) >= 8
+    assert sum(1 for l in lines if l.startswith("QE ")) >= 3
+    assert "EPS ok" in out.stdout
+    assert "mu = nu => G(site, mu, nu) = 0" in out.stderr

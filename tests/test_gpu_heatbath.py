@@ -32,6 +32,27 @@ def test_plaquette_at_known_couplings(beta, lo, hi):
     assert lo < P < hi and du < 1e-13 and dd < 1e-13
 
 
+def test_plaquette_at_beta_6_with_error():
+    """<P> at beta = 6.0 on 16^4: 100 thermalisation sweeps from a hot start,
+    then the plaquette after each of 400 sweeps; the error from 20 bins of 20
+    sweeps.  The infinite-volume value is 0.59368; finite-volume effects at
+    16^4 are below 1e-4."""
+    L = (16, 16, 16, 16)
+    with qcbflow.GradientFlow(L) as gf:
+        gf.hot_start(3)
+        gf.heatbath(6.0, 100, 2024)
+        ps = []
+        for _ in range(400):
+            gf.heatbath(6.0, 1, 2024)
+            ps.append(gf.plaquette())
+    ps = np.array(ps)
+    bins = ps.reshape(20, 20).mean(axis=1)
+    P, err = ps.mean(), bins.std(ddof=1) / np.sqrt(len(bins))
+    say(f"  beta 6.0, 16^4: <P> = {P:.6f} +- {err:.6f} (20 bins of 20 sweeps)")
+    assert err < 5e-5
+    assert abs(P - 0.59368) < 4 * err + 1e-4
+
+
 def test_agrees_with_oracle_heatbath():
     L = (6, 6, 6, 6)
     f = oracle.thermalised(L, 6.0, 1234, 40)

@@ -101,6 +101,23 @@ def test_roundtrip_across_transfer_chunks(tmp_path):
         assert np.abs(gm.field() - f).max() < 1e-6
 
 
+def test_force_readback_across_transfer_chunks():
+    """qcbf_get_force streams through the 65536-site staging chunk: 80640
+    sites, the last chunk partial, both force slots, against the oracle's
+    force (and force_1 equal to force_0 of the same field)."""
+    L = (20, 18, 16, 14)
+    say(f"force read-back {L} across chunks")
+    f = oracle.hot_start(L, 31)
+    ref = oracle.compute_force(f.copy(), L)
+    with qcbflow.GradientFlow(L) as gf:
+        gf.copy(f)
+        gf.compute_force(0)
+        gf.compute_force(1)
+        f0, f1 = gf.force(0), gf.force(1)
+    assert np.abs(f0 - ref).max() < 1e-13
+    assert np.array_equal(f0, f1)
+
+
 def test_load_single_precision_payload(tmp_path):
     L = (4, 4, 4, 4)
     f = oracle.hot_start(L, 6)

@@ -87,3 +87,21 @@ def test_heatbath_generator_thermalises():
     U = fields.to_cplx(f).reshape(-1, 3, 3)
     assert np.abs(U @ np.conj(np.swapaxes(U, 1, 2)) - np.eye(3)).max() < 1e-13
     assert np.abs(np.linalg.det(U) - 1).max() < 1e-13
+
+
+def test_local_q_e_closed_form_and_sums():
+    """local_q_e_clover per site: on a constant abelian field every site carries
+    q = Q/V and e = E (closed forms); on a random field the site values sum to
+    the lattice totals of topo_q_e_clover."""
+    L = (4, 6, 4, 8)
+    f, phis = fields.abelian_flux(L, [1, 2, -1, 1, 3, 2])
+    P, E, Q = fields.abelian_expected(L, phis)
+    V = int(np.prod(L))
+    for site in (0, 7, V // 2 + 3, V - 1):
+        q, e = oracle.local_q_e(f, L, site)
+        assert abs(q - Q / V) < 1e-14 * max(1.0, abs(Q / V)) and abs(e - E) < 1e-13 * E
+    L = (4, 4, 4, 4)
+    g = oracle.hot_start(L, 3)
+    Qt, Et = oracle.q_e_clover(g, L)
+    qe = np.array([oracle.local_q_e(g, L, s) for s in range(256)])
+    assert abs(qe[:, 0].sum() - Qt) < 1e-12 and abs(qe[:, 1].mean() - Et) < 1e-12 * Et
