@@ -133,7 +133,6 @@ hipError_t launch_measure(prec p, const void* U, const lattice& g, double* parti
                           : measure_launch<prec::f32m>(U, g, partials, out, clover, st);
 }
 
-// ---- unitarity check --------------------------------------------------------
 // ---- one site -----------------------------------------------------------------
 // The link U_d(s) in FP64 whatever the storage form.
 template <class F>
@@ -256,6 +255,7 @@ hipError_t launch_local_q_e(prec p, const void* U, const lattice& g, unsigned si
     return hipGetLastError();
 }
 
+// ---- unitarity check --------------------------------------------------------
 template <prec P>
 __global__ __launch_bounds__(256) void k_unitarity(const typename storage<P>::vec* __restrict__ U,
                                                    lattice g, double* __restrict__ partials) {

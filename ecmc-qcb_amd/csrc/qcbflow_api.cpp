@@ -76,6 +76,8 @@ static int fail(qcbf_flow* h, int code, const char* fmt, ...) {
     return code;
 }
 
+int qcb::fail_without_handle(int code, const char* msg) { return fail(nullptr, code, "%s", msg); }
+
 #define HIPCHK(h, expr)                                                                     \
     do {                                                                                    \
         hipError_t e_ = (expr);                                                             \

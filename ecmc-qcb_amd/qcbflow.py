@@ -108,6 +108,7 @@ SIGNATURES = {
     "qcbf_timing_read": (ctypes.c_int, [_vp, _dp, _lp]),
     "qcbf_copy_bench": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_int, _dp]),
     "qcbf_device_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, _ip]),
+    "qcbf_measure_device": (ctypes.c_int, [_vp, _ip, ctypes.c_int, _vp, _vp]),
 }
 
 _lib = None

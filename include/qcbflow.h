@@ -190,6 +190,30 @@ int qcbf_synchronize(qcbf_flow* h);
  * *flagged (may be NULL) is set to 1 if any had t > 1. */
 int qcbf_su3_exp(const double* Z, int n, double coeff, double* out, int* flagged);
 
+/* ---- allocation-free measurement (no counterpart in the reference) ------- */
+/* The measurement of qcbf_measure (same kernels, same results bit for bit) on
+ * a link field the caller already holds in device memory, for callers that
+ * cannot allocate device memory: code under stream capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  Needs no handle and allocates
+ * nothing: its per-tile partial sums live in a fixed device array of the
+ * library, 2^18 tiles (6 MiB per GPU, kept until the process ends), so the
+ * volume is limited to 2^24 sites (64^4); larger volumes are rejected with
+ * QCBF_ERR_ARG.  Calls that overlap on one GPU share that array and race.
+ *   links: the field in the library's device layout for `precision`, with
+ *          V = Lx*Ly*Lz*Lt and site s = ((t*Lz + z)*Ly + y)*Lx + x:
+ *          QCBF_FP64: entry k = 3i+j of U_mu(s) as (re, im) doubles at
+ *            double2 index (mu*9 + k)*V + s (rows 0 and 1 are read; the
+ *            third row is rebuilt as conj(row0 x row1));
+ *          QCBF_FP32_MIXED: rows 0-1 as three float4 per link at float4
+ *            index (mu*3 + w)*V + s, w = 0: (U00, U01), 1: (U02, U10),
+ *            2: (U11, U12), each complex as (re, im).
+ *   out:   device memory for 3 doubles, {P, E, Q} in the conventions above.
+ *   stream: a hipStream_t (NULL: the default stream); the call is
+ *          asynchronous on it.
+ * Errors are read with qcbf_last_error(NULL). */
+int qcbf_measure_device(const void* links, const int dims[4], int precision, double* out,
+                        void* stream);
+
 /* ---- launch tuning (no counterpart in the reference) --------------------- */
 enum qcbf_tune_key {
     QCBF_TUNE_MAP_MODE = 0, /* workgroup -> site-tile order of the flow kernels:
