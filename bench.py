@@ -75,15 +75,18 @@ def log(msg):
 
 
 def traffic_from_profiles(lattice):
-    """HBM bytes per stage launch from the committed rocprofv3 PMC summary of
-    this workload (profiles/rk3_stage_f64_<lattice>_traffic.md), or None."""
+    """HBM bytes per FP64 stage launch from the committed rocprofv3 PMC summary
+    of this workload (profiles/rk3_stage_f64_<lattice>_traffic.md or
+    profiles/rk3_stage_<lattice>_kernel_trace_traffic.md), or None."""
     tag = "x".join(str(v) for v in lattice)
-    path = os.path.join(ROOT, "profiles", f"rk3_stage_f64_{tag}_traffic.md")
-    if not os.path.exists(path):
-        return None
-    with open(path) as f:
-        m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read())
-    return float(m.group(1)) if m else None
+    for name in (f"rk3_stage_f64_{tag}_traffic.md", f"rk3_stage_{tag}_kernel_trace_traffic.md"):
+        path = os.path.join(ROOT, "profiles", name)
+        if os.path.exists(path):
+            with open(path) as f:
+                m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read())
+            if m:
+                return float(m.group(1))
+    return None
 
 
 def host_cpus():
