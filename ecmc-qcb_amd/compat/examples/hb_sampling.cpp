@@ -9,9 +9,11 @@
 // that it can be flowed again elsewhere; the (t, Q, E) series is printed at
 // full precision and appended to the reference's "t Q E" text file.
 //
-// usage: hb_sampling <run_dir> <run_name> <L> <beta> <seed> <N_therm> <N_shift>
+// usage: hb_sampling <run_dir> <run_name> <L_core> <beta> <seed> <N_therm> <N_shift>
 //                    <N_switch_eo> <N_sweeps> <N_shift_plaquette> <N_shift_topo>
-//                    <N_steps_gf> <N_rk_steps>
+//                    <N_steps_gf> <N_rk_steps> [<n_core_dims>]
+// (the lattice is n_core_dims L_core sites per dimension; n_core_dims
+// defaults to 1)
 #include <cstdio>
 #include <cstdlib>
 #include <random>
@@ -30,11 +32,11 @@
 #include "../observables/observables_mpi.h"
 
 int main(int argc, char* argv[]) {
-    if (argc != 14) {
+    if (argc != 14 && argc != 15) {
         std::fprintf(stderr,
-                     "usage: %s <run_dir> <run_name> <L> <beta> <seed> <N_therm> <N_shift> "
+                     "usage: %s <run_dir> <run_name> <L_core> <beta> <seed> <N_therm> <N_shift> "
                      "<N_switch_eo> <N_sweeps> <N_shift_plaquette> <N_shift_topo> <N_steps_gf> "
-                     "<N_rk_steps>\n",
+                     "<N_rk_steps> [<n_core_dims>]\n",
                      argv[0]);
         return 2;
     }
@@ -42,7 +44,7 @@ int main(int argc, char* argv[]) {
     rp.run_dir = argv[1];
     rp.run_name = argv[2];
     rp.L_core = std::atoi(argv[3]);
-    rp.n_core_dims = 1;
+    rp.n_core_dims = argc == 15 ? std::atoi(argv[14]) : 1;
     rp.hp.beta = std::atof(argv[4]);
     rp.seed = std::atoi(argv[5]);
     rp.N_therm = std::atoi(argv[6]);

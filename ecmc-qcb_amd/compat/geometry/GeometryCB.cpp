@@ -1,9 +1,13 @@
 #include "GeometryCB.h"
 
+#include "../mpi/MpiTopology.h"
+
 // Neighbour table over the extended (L+2)^4 box; links that would leave the
-// box are SIZE_MAX, as in the reference (GeometryCB.cpp:10-31).
+// box are SIZE_MAX, as in the reference (GeometryCB.cpp:10-31).  L_ is the
+// caller's L_core; on one GPU the box holds all n_core_dim^4 sub-domains
+// (mpi::one_rank_fold).
 GeometryCB::GeometryCB(int L_) {
-    L_int = L_;
+    L_int = L_ * mpi::one_rank_fold();
     L_ext = L_int + 2;
     V_int = static_cast<size_t>(L_int) * L_int * L_int * L_int;
     V_ext = static_cast<size_t>(L_ext) * L_ext * L_ext * L_ext;

@@ -24,3 +24,19 @@ def test_compat_halos_are_periodic_images(tmp_path):
     print(r.stdout)
     assert r.returncode == 0, r.stdout
     assert "wrong: 0" in r.stdout
+
+
+@pytest.mark.skipif(CXX is None, reason="g++ not installed")
+def test_compat_topology_folds_sub_domains_into_one_lattice(tmp_path):
+    """MpiTopology(n_core_dim = 2) then GeometryCB(L_core = 4): one 8^4 lattice
+    on the one rank (the reference's 16 sub-domains of 4^4), as the reference's
+    own input files ask for."""
+    exe = tmp_path / "compat_fold_check"
+    srcs = [os.path.join(COMPAT, "geometry/GeometryCB.cpp")]
+    subprocess.run([CXX, "-O1", "-std=c++20", "-I", COMPAT,
+                    os.path.join(ROOT, "tests", "compat_fold_check.cpp"), *srcs, "-o", str(exe)],
+                   check=True, capture_output=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0 and "wrong: 0" in r.stdout, r.stdout
+    assert "n_core_dim = 2 on one GPU" in r.stderr

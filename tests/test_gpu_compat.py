@@ -135,6 +135,26 @@ def test_sampling_loop_caller_matches_oracle_on_evolving_configurations(tmp_path
     assert len(topo) == 33
 
 
+def test_sampling_loop_with_sub_domains_runs_them_as_one_lattice(tmp_path):
+    """The reference's input files ask for n_core_dims = 2 (16 sub-domains of
+    L_core^4): on one GPU the drop-in runs them as one lattice of 2 L_core
+    per dimension, so the loop with L_core = 4, n_core_dims = 2 prints exactly
+    what it prints for one 8^4 lattice."""
+    print("compat hb_sampling: 8^4 as L_core 8 and as 2^4 sub-domains of 4^4", flush=True)
+    outs = []
+    for lc, n in ((8, 1), (4, 2)):
+        d = tmp_path / f"L{lc}n{n}"
+        d.mkdir()
+        args = [SAMPLING, str(d), "run", str(lc), "6.0", "77", "2", "3", "2", "1", "1", "2",
+                "6", "2", str(n)]
+        out = subprocess.run(args, capture_output=True, text=True, timeout=300)
+        assert out.returncode == 0, out.stderr[-2000:]
+        outs.append([l for l in out.stdout.splitlines() if l.startswith(("tQE ", "P "))])
+        if n == 2:
+            assert "n_core_dim = 2 on one GPU" in out.stderr
+    assert len(outs[0]) == 3 + 2 * 6 and outs[0] == outs[1]
+
+
 def test_compat_observables_per_site_and_levi_civita(tmp_path):
     """clover_site, local_q_e_clover and levi_civita of the drop-in
     (src/observables/observables_mpi.h:8-19) against the oracle, through a
