@@ -563,7 +563,7 @@ def headline(qcbflow, args):
         log(f"fp32-mixed: {mixed['value']:.4e} site-steps/s ({mixed['vs_fp64']:.2f}x fp64), "
             f"stage {mstage:.4f} ms")
         gm.close()
-        # the north_star gates at t = 2, on a leg of its own (2/eps steps)
+        # the precision gates at t = 2 (FP32-mixed vs FP64), on a leg of its own (2/eps steps)
         acc = accuracy_leg(qcbflow, src, L, args.eps, every)
         mixed["accuracy_vs_fp64"] = acc
         log(f"fp32-mixed vs fp64 at t={acc['t_gate']:.2f} ({acc['rk3_steps']} steps): "

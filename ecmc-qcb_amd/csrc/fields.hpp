@@ -77,10 +77,21 @@ __device__ inline void put4(float4* p, float4 v) {
 #endif
 }
 
+// 1/sqrt(n) in FP64 for the squared row norms of a link (n within FP32's
+// normal range; about 1 for SU(3) rows): the hardware's FP32 reciprocal square
+// root as the seed (relative error about 1e-7) and one FP64 Newton step
+// y (3 - n y^2) / 2, relative error about 1e-14, far below the FP32 rounding
+// of the stored rows.  A correctly rounded 1.0 / sqrt(n) costs an FP64 divide
+// and square-root sequence per row.
+__device__ inline double rsqrt_rows(double n) {
+    const double y = (double)__builtin_amdgcn_rsqf((float)n);
+    return y * (1.5 - 0.5 * n * y * y);
+}
+
 // FP64 Gram-Schmidt of two rows: r0 <- r0/|r0|, r1 <- (r1 - <r0,r1> r0)/|.|
 __device__ inline void orthonormalise_rows(cpx<double> (&r0)[3], cpx<double> (&r1)[3]) {
     double n0 = norm2(r0[0]) + norm2(r0[1]) + norm2(r0[2]);
-    const double i0 = 1.0 / sqrt(n0);
+    const double i0 = rsqrt_rows(n0);
 #pragma unroll
     for (int k = 0; k < 3; k++) r0[k] = i0 * r0[k];
     cpx<double> dot = {0.0, 0.0};
@@ -89,7 +100,7 @@ __device__ inline void orthonormalise_rows(cpx<double> (&r0)[3], cpx<double> (&r
 #pragma unroll
     for (int k = 0; k < 3; k++) r1[k] = r1[k] - dot * r0[k];
     const double n1 = norm2(r1[0]) + norm2(r1[1]) + norm2(r1[2]);
-    const double i1 = 1.0 / sqrt(n1);
+    const double i1 = rsqrt_rows(n1);
 #pragma unroll
     for (int k = 0; k < 3; k++) r1[k] = i1 * r1[k];
 }
