@@ -459,6 +459,12 @@ def report_ensemble(qcbflow, args):
             f"identical to single-lattice path: {same}")
         e.close()
     out["fp32_mixed_vs_fp64"] = out["fp32_mixed"]["value"] / out["fp64"]["value"]
+    # the FP32-mixed batched flow against the plain FP64 path (the same 64
+    # configurations flowed one at a time by the single-lattice FP64 kernels)
+    out["fp32_mixed_batched_vs_fp64_plain"] = (out["fp32_mixed"]["value"] /
+                                               out["fp64"]["sequential_value"])
+    log(f"  fp32-mixed batched vs fp64 plain (one lattice at a time): "
+        f"{out['fp32_mixed_batched_vs_fp64_plain']:.2f}x")
     per = [compare(res["fp64"][0][c], res["fp64"][1][c], res["fp32_mixed"][0][c],
                    res["fp32_mixed"][1][c]) for c in range(64)]
     keys = ("dE_rel_at_t_gate", "dQ_abs_at_t_gate", "dP_rel_at_t_gate", "max_dE_rel",
