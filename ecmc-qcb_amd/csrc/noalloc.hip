@@ -42,10 +42,13 @@ extern "C" int qcbf_measure_device(const void* links, const int dims[4], int pre
             return fail(QCBF_ERR_ARG, "qcbf_measure_device: extent out of range [2, 4096]");
         V *= (unsigned long long)dims[d];
     }
-    const qcb::lattice g = qcb::make_lattice(dims);
-    if (V >= (1ull << 28) || qcb::measure_blocks(g) > kNoAllocMaxTiles)
+    if (V > (1ull << 24))
         return fail(QCBF_ERR_ARG,
                     "qcbf_measure_device: more than 2^24 sites (its scratch holds 2^18 tiles)");
+    const qcb::lattice g = qcb::make_lattice(dims);
+    if (qcb::measure_blocks(g) > kNoAllocMaxTiles)
+        return fail(QCBF_ERR_ARG,
+                    "qcbf_measure_device: more than 2^18 tiles (its scratch holds 2^18 tiles)");
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
         return fail(QCBF_ERR_HIP, "qcbf_measure_device: no usable current device");
