@@ -46,6 +46,23 @@ def test_mixed_roundtrip_is_su3_to_fp32():
     assert du < 1e-6 and dd < 1e-6
 
 
+def test_mixed_upload_reunitarises_scaled_links():
+    """The FP32-mixed upload re-unitarises the two stored rows (FP64 row
+    Gram-Schmidt with a Newton-refined reciprocal square root): links scaled
+    by 1.5 and 0.25 (squared row norms 2.25 and 0.0625, far from 1) come back
+    as the SU(3) links they were scaled from, to FP32 rounding."""
+    L = (4, 4, 4, 8)
+    f = oracle.hot_start(L, 11)
+    for scale in (1.5, 0.25):
+        with qcbflow.GradientFlow(L, precision=qcbflow.FP32_MIXED) as gf:
+            gf.copy(f * scale)
+            g = gf.field()
+            du, dd = gf.unitarity()
+        say(f"scale {scale}: max |g - f| {np.abs(g - f).max():.2e}, unitarity {du:.2e} {dd:.2e}")
+        assert np.abs(g - f).max() < 2e-7
+        assert du < 1e-6 and dd < 1e-6
+
+
 def test_mixed_measure_closed_form():
     L = (4, 6, 4, 8)
     f, phis = fields.abelian_flux(L, [1, 2, -1, 1, 3, 2])
