@@ -14,7 +14,10 @@ Beside it: the FP32-mixed form on the same input and cadence (`fp32_mixed`),
 its accuracy against FP64 at flow time t = 2 from a separate untimed leg of
 2/eps steps in both precisions (whatever K is), the roofline object of the
 dominant kernel (the plain RK3 stage kernel, every launch of the timed
-region bracketed by HIP events on the handle's stream) and the CPU baseline
+region bracketed by HIP events on the handle's stream), the batched form on
+its own workload (`batched_ensemble`: the 64 lattices of uneven volumes of
+`--report ensemble`, about 10 s, with the FP32-mixed batched flow against the
+plain FP64 path flowing the same lattices one at a time) and the CPU baseline
 (the FP64 oracle, a C restatement of the reference flow, on the host cores:
 one RK3 step and one measurement of the same field, extrapolated).
 
@@ -476,6 +479,35 @@ def report_ensemble(qcbflow, args):
     return out
 
 
+def ensemble_leg(qcbflow, args):
+    """The 64-lattice ensemble workload (report_ensemble with its own defaults:
+    150 steps, 30 sweeps, autotuned) condensed for the bench line: the
+    north_star's comparison of the FP32-mixed batched flow with the plain FP64
+    path, and the precision gates over all 64 configurations."""
+    ea = argparse.Namespace(**vars(args))
+    ea.steps_given, ea.sweeps, ea.threads_per_site, ea.no_autotune = False, None, None, False
+    r = report_ensemble(qcbflow, ea)
+    acc = r["accuracy_fp32_mixed_vs_fp64"]
+    return {
+        "workload": r["config"]["workload"] + " (hipGraph-replayed steps, autotuned launch shape)",
+        "sum_sites": r["config"]["sum_sites"],
+        "fp64_value": r["fp64"]["value"],
+        "fp32_mixed_value": r["fp32_mixed"]["value"],
+        "fp64_plain_one_at_a_time_value": r["fp64"]["sequential_value"],
+        "fp32_mixed_plain_one_at_a_time_value": r["fp32_mixed"]["sequential_value"],
+        "fp32_mixed_batched_vs_fp64_plain": r["fp32_mixed_batched_vs_fp64_plain"],
+        "fp32_mixed_vs_fp64_batched": r["fp32_mixed_vs_fp64"],
+        "bit_identical_to_single_lattice_path": bool(
+            r["fp64"]["bit_identical_to_single_lattice_path"]
+            and r["fp32_mixed"]["bit_identical_to_single_lattice_path"]),
+        "launch_shape": {"fp64": r["fp64"]["launch_shape"],
+                         "fp32_mixed": r["fp32_mixed"]["launch_shape"]},
+        "accuracy_fp32_mixed_vs_fp64": acc,
+        "gate_dE_lt_1e-6": acc["dE_rel_at_t_gate"] is not None and acc["dE_rel_at_t_gate"] < 1e-6,
+        "gate_dQ_lt_1e-4": acc["dQ_abs_at_t_gate"] is not None and acc["dQ_abs_at_t_gate"] < 1e-4,
+    }
+
+
 def headline(qcbflow, args):
     """The single-lattice workload of the bench line (a preset, by default the
     64^4 lattice measured every 10 steps): FP64 timed leg (value), FP32-mixed
@@ -577,6 +609,11 @@ def headline(qcbflow, args):
             f"dP/P {acc['dP_rel_at_t_gate']:.2e}")
     src.close()
 
+    # the batched form on its own workload (64 lattices of uneven volumes)
+    ensemble = None
+    if not args.no_mixed and not args.no_ensemble and args.L is None and args.preset == "large":
+        ensemble = ensemble_leg(qcbflow, args)
+
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
         nproc, model = host_cpus()
@@ -644,6 +681,7 @@ def headline(qcbflow, args):
         "host_transfer_ms": {"up": t_up * 1e3, "down": t_down * 1e3},
         "device_memory": device_memory(V, "fp64"),
         "fp32_mixed": mixed,
+        "batched_ensemble": ensemble,
         "cpu_baseline": cpu,
         "copy_calibration": {"mib": copy_mib, "avg_ms": copy_ms},
     }
@@ -672,6 +710,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-mixed legs")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the 64-lattice ensemble leg of the default bench line")
     ap.add_argument("--time-every", type=int, default=1,
                     help="time every n-th launch of each kernel class with HIP events")
     ap.add_argument("--map-mode", type=int, default=None,
