@@ -300,16 +300,18 @@ __host__ __device__ constexpr horner_coeffs make_horner_coeffs() {
 }
 
 // Cayley-Hamilton coefficients of exp(X) = q0 1 + q1 X + q2 X^2 for X in su(3)
-// with t = -1/2 tr X^2 and d = Re(i det X), from the fixed 18-step Horner
-// recursion of exp_su3_luscher (src/su3/utils.cpp:168-186):
-//   q <- (c_n - i d q2, q0 - t q2, q1),  n = 17..0.
+// with t = -1/2 tr X^2 and d = Re(i det X), by the Horner recursion of
+// exp_su3_luscher (src/su3/utils.cpp:168-186) from the term n = N, i.e. for
+// the Taylor polynomial of degree N (the reference's N is 18):
+//   q <- (c_n - i d q2, q0 - t q2, q1),  n = N-1..0.
 // Evaluated in the type H (double in every path).
-template <typename H>
-__host__ __device__ inline void horner18(H t, H d, cpx<H>& q0, cpx<H>& q1, cpx<H>& q2) {
+template <int N, typename H>
+__host__ __device__ inline void horner_n(H t, H d, cpx<H>& q0, cpx<H>& q1, cpx<H>& q2) {
+    static_assert(N >= 1 && N <= 18, "degree 1..18");
     constexpr horner_coeffs hc = make_horner_coeffs();
-    cpx<H> a0 = {H(hc.c[18]), H(0)}, a1 = {H(0), H(0)}, a2 = {H(0), H(0)};
+    cpx<H> a0 = {H(hc.c[N]), H(0)}, a1 = {H(0), H(0)}, a2 = {H(0), H(0)};
 #pragma unroll
-    for (int n = 17; n >= 0; n--) {
+    for (int n = N - 1; n >= 0; n--) {
         cpx<H> n0 = {H(hc.c[n]) + d * a2.i, -d * a2.r};  // c_n - i d q2
         cpx<H> n1 = {a0.r - t * a2.r, a0.i - t * a2.i};   // q0 - t q2
         a2 = a1;
@@ -319,6 +321,26 @@ __host__ __device__ inline void horner18(H t, H d, cpx<H>& q0, cpx<H>& q1, cpx<H
     q0 = a0;
     q1 = a1;
     q2 = a2;
+}
+// Below this t the terms n = 11..18 of the 18-term series change exp(X) by
+// less than 2^-56 (an eighth of an FP64 ulp of 1): |X| <= |X|_F = sqrt(2t) <
+// 0.1415 and sum_{n>10} |X|^n/n! < |X|^11/11! e^|X| < 1.2e-17.  The
+// exponents of a Wilson-flow step at eps = 0.02 stay below t = 2e-3, hot
+// starts included.
+#ifndef QCB_HORNER_SHORT
+#define QCB_HORNER_SHORT 1
+#endif
+constexpr double kHornerShortT = 0.01;
+
+// The coefficients of the 18-term series, evaluated as the degree-10
+// polynomial where that is equal to it within 2^-56 (per lane, so the
+// choice never depends on which sites share a wavefront).
+template <typename H>
+__host__ __device__ inline void horner_exp(H t, H d, cpx<H>& q0, cpx<H>& q1, cpx<H>& q2) {
+    if (QCB_HORNER_SHORT && t < H(kHornerShortT))
+        horner_n<10>(t, d, q0, q1, q2);
+    else
+        horner_n<18>(t, d, q0, q1, q2);
 }
 
 // exp(X) for X in su(3), following exp_su3_luscher (src/su3/utils.cpp:145-193).
@@ -355,7 +377,7 @@ __host__ __device__ inline mat3<T> exp_su3(const alg8<T>& x, bool* warn) {
            H(2) * (pri * H(q.r) - prr * H(q.i));
     if (warn && tt > H(1)) *warn = true;
     cpx<H> h0, h1, h2;
-    horner18<H>(tt, dd, h0, h1, h2);
+    horner_exp<H>(tt, dd, h0, h1, h2);
     const cpx<T> q0 = {T(h0.r), T(h0.i)}, q1 = {T(h1.r), T(h1.i)}, q2 = {T(h2.r), T(h2.i)};
 
     mat3<T> E;
