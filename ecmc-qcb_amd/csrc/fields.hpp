@@ -7,11 +7,13 @@
 //          per link, U[(mu*3 + w)*V + s]:
 //            w = 0: (U00, U01)   w = 1: (U02, U10)   w = 2: (U11, U12);
 //          the third row is rebuilt on load as conj(row0 x row1), which is
-//          exact for SU(3) (U^dag = adj U when det U = 1).  Stores
-//          re-unitarise the two rows in FP64 by row Gram-Schmidt (the
-//          reference's proj_su3, src/su3/utils.cpp:194-214, does the same by
-//          columns on full matrices; with two stored rows the row form is the
-//          natural one), so the stored field stays in SU(3) to FP32 rounding.
+//          exact for SU(3) (U^dag = adj U when det U = 1).  Uploads and the
+//          flow's RK3 stage 3 re-unitarise the two rows in FP64 by row
+//          Gram-Schmidt (the reference's proj_su3, src/su3/utils.cpp:194-214,
+//          does the same by columns on full matrices; with two stored rows the
+//          row form is the natural one), so the field stays in SU(3) to FP32
+//          rounding; the intermediate fields of stages 1 and 2 keep the FP32
+//          rows of exp(X) U, within a few FP32 roundings of SU(3).
 // The FP64 flow output may hold rows 0-1 only (store_flow_link64,
 // lattice.hpp); the FP32-mixed form never stores the third row.
 // su(3) fields (force / RK3 accumulator) hold the 8 reals of alg8:
@@ -25,6 +27,12 @@
 // rows and a cross product (mul_su3 and friends, su3.hpp).  0 = full products.
 #ifndef QCB_SU3_PRODUCTS
 #define QCB_SU3_PRODUCTS 1
+#endif
+
+// FP32-mixed flow: re-unitarise the stored rows after every RK3 stage (1) or
+// after stage 3 only (0).
+#ifndef QCB_F32M_PROJECT_EVERY_STAGE
+#define QCB_F32M_PROJECT_EVERY_STAGE 0
 #endif
 
 namespace qcb {
@@ -127,6 +135,15 @@ __device__ inline void store_link32c(float4* __restrict__ p, unsigned V, int mu,
     store_rows32c(p, V, mu, s, r0, r1);
 }
 
+// Rows 0-1 of an FP32 link stored as they are (no re-unitarisation).
+__device__ inline void store_link32c_raw(float4* __restrict__ p, unsigned V, int mu, unsigned s,
+                                         const mat3<float>& m) {
+    float4* b = p + (size_t)mu * 3 * V + s;
+    put4(b, make_float4(m.e[0][0].r, m.e[0][0].i, m.e[0][1].r, m.e[0][1].i));
+    put4(b + V, make_float4(m.e[0][2].r, m.e[0][2].i, m.e[1][0].r, m.e[1][0].i));
+    put4(b + 2 * (size_t)V, make_float4(m.e[1][1].r, m.e[1][1].i, m.e[1][2].r, m.e[1][2].i));
+}
+
 __device__ inline alg8<float> load_alg32(const float4* __restrict__ p, unsigned V, int mu,
                                          unsigned s) {
     const float4* b = p + (size_t)mu * 2 * V + s;
@@ -166,6 +183,11 @@ struct storage<prec::f64> {
     __device__ static void store(vec* p, unsigned V, int mu, unsigned s, const mat3<double>& m) {
         store_flow_link64(p, V, mu, s, m);
     }
+    // the output of RK3 stage `stage` (the same for every stage in FP64)
+    __device__ static void store_stage(vec* p, unsigned V, int mu, unsigned s,
+                                       const mat3<double>& m, int) {
+        store_flow_link64(p, V, mu, s, m);
+    }
     __device__ static alg8<double> load_alg(const avec* p, unsigned V, int mu, unsigned s) {
         return load_alg64(p, V, mu, s);
     }
@@ -186,6 +208,16 @@ struct storage<prec::f32m> {
     __device__ static field view(const vec* p, unsigned V) { return field{p, V}; }
     __device__ static void store(vec* p, unsigned V, int mu, unsigned s, const mat3<float>& m) {
         store_link32c(p, V, mu, s, m);
+    }
+    // the output of RK3 stage `stage`: re-unitarised in FP64 after stage 3
+    // (and after every stage with QCB_F32M_PROJECT_EVERY_STAGE); the
+    // intermediate fields of stages 1 and 2 keep the FP32 rows of exp(X) U.
+    __device__ static void store_stage(vec* p, unsigned V, int mu, unsigned s,
+                                       const mat3<float>& m, int stage) {
+        if (QCB_F32M_PROJECT_EVERY_STAGE || stage == 3)
+            store_link32c(p, V, mu, s, m);
+        else
+            store_link32c_raw(p, V, mu, s, m);
     }
     __device__ static alg8<float> load_alg(const avec* p, unsigned V, int mu, unsigned s) {
         return load_alg32(p, V, mu, s);

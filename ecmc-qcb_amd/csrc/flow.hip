@@ -261,7 +261,7 @@ __global__ __launch_bounds__(256) void k_update(int which, typename storage<P>::
     bool w = false;
     const mat3<T> e = exp_su3<T, double>(x, &w);
     const mat3<T> u = S::view(U, g.V).link(mu, s);
-    S::store(U, g.V, mu, s, mul(e, u));
+    S::store_stage(U, g.V, mu, s, mul(e, u), which);
     if (w) atomicOr(warn, 1);
 }
 

@@ -231,7 +231,7 @@ __device__ inline void stage_update(const alg8<typename storage<P>::real>& z,
         }
     }
     const mat3<T> e = exp_su3<T, double>(x, &warn);
-    S::store(Uout, V, MU, s, mul(e, u));
+    S::store_stage(Uout, V, MU, s, mul(e, u), STAGE);
 }
 
 template <prec P, int STAGE, int MU, bool XROWS = false, class Hook = no_staple_hook>
