@@ -40,6 +40,22 @@ namespace qcb {
 enum class prec { f64 = 0, f32m = 1 };
 
 // ---- FP32 two-row compressed field -------------------------------------------
+// Word s of a word array of the field (16-byte words).  The array's base is
+// the same for the whole wavefront (scalar registers) and the site enters as a
+// 32-bit byte offset, so each load takes both as they are instead of a 64-bit
+// address formed per load.  Needs 16 s < 2^32: fields of at most 2^28 sites
+// (kMaxSites32c; qcbf_create and qcbf_batch_create take volumes below it).
+constexpr unsigned kMaxSites32c = 1u << 28;
+template <class W>
+__device__ inline W* word_ptr(W* base, unsigned s) {
+    static_assert(sizeof(W) == 16, "16-byte words");
+    using B = typename std::conditional<std::is_const<W>::value, const char, char>::type;
+    return reinterpret_cast<W*>(reinterpret_cast<B*>(base) + (s << 4));
+}
+__device__ inline float4 word_at(const float4* __restrict__ base, unsigned s) {
+    return *word_ptr(base, s);
+}
+
 struct field32c {
     using real = float;
     using word = float4;
@@ -48,10 +64,8 @@ struct field32c {
     const float4* __restrict__ p;
     unsigned V;
     __device__ inline void load_words(int mu, unsigned s, word (&w)[kWords]) const {
-        const float4* b = p + (size_t)mu * 3 * V + s;
-        w[0] = b[0];
-        w[1] = b[V];
-        w[2] = b[2 * (size_t)V];
+#pragma unroll
+        for (int k = 0; k < kWords; k++) w[k] = word_at(p + ((size_t)mu * 3 + k) * V, s);
     }
     __device__ static inline mat3<float> from_words(const word (&w)[kWords]) {
         const float4 a = w[0], c = w[1], d = w[2];
@@ -116,10 +130,12 @@ __device__ inline void orthonormalise_rows(cpx<double> (&r0)[3], cpx<double> (&r
 __device__ inline void store_rows32c(float4* __restrict__ p, unsigned V, int mu, unsigned s,
                                      cpx<double> (&r0)[3], cpx<double> (&r1)[3]) {
     orthonormalise_rows(r0, r1);
-    float4* b = p + (size_t)mu * 3 * V + s;
-    put4(b, make_float4((float)r0[0].r, (float)r0[0].i, (float)r0[1].r, (float)r0[1].i));
-    put4(b + V, make_float4((float)r0[2].r, (float)r0[2].i, (float)r1[0].r, (float)r1[0].i));
-    put4(b + 2 * (size_t)V,
+    float4* b = p + (size_t)mu * 3 * V;
+    put4(word_ptr(b, s),
+         make_float4((float)r0[0].r, (float)r0[0].i, (float)r0[1].r, (float)r0[1].i));
+    put4(word_ptr(b + V, s),
+         make_float4((float)r0[2].r, (float)r0[2].i, (float)r1[0].r, (float)r1[0].i));
+    put4(word_ptr(b + 2 * (size_t)V, s),
          make_float4((float)r1[1].r, (float)r1[1].i, (float)r1[2].r, (float)r1[2].i));
 }
 
@@ -138,32 +154,33 @@ __device__ inline void store_link32c(float4* __restrict__ p, unsigned V, int mu,
 // Rows 0-1 of an FP32 link stored as they are (no re-unitarisation).
 __device__ inline void store_link32c_raw(float4* __restrict__ p, unsigned V, int mu, unsigned s,
                                          const mat3<float>& m) {
-    float4* b = p + (size_t)mu * 3 * V + s;
-    put4(b, make_float4(m.e[0][0].r, m.e[0][0].i, m.e[0][1].r, m.e[0][1].i));
-    put4(b + V, make_float4(m.e[0][2].r, m.e[0][2].i, m.e[1][0].r, m.e[1][0].i));
-    put4(b + 2 * (size_t)V, make_float4(m.e[1][1].r, m.e[1][1].i, m.e[1][2].r, m.e[1][2].i));
+    float4* b = p + (size_t)mu * 3 * V;
+    put4(word_ptr(b, s), make_float4(m.e[0][0].r, m.e[0][0].i, m.e[0][1].r, m.e[0][1].i));
+    put4(word_ptr(b + V, s), make_float4(m.e[0][2].r, m.e[0][2].i, m.e[1][0].r, m.e[1][0].i));
+    put4(word_ptr(b + 2 * (size_t)V, s),
+         make_float4(m.e[1][1].r, m.e[1][1].i, m.e[1][2].r, m.e[1][2].i));
 }
 
 __device__ inline alg8<float> load_alg32(const float4* __restrict__ p, unsigned V, int mu,
                                          unsigned s) {
-    const float4* b = p + (size_t)mu * 2 * V + s;
+    const float4* b = p + (size_t)mu * 2 * V;
 #if QCB_NT_ACC_LOADS
-    const float* q0 = &b[0].x;
-    const float* q1 = &b[V].x;
+    const float* q0 = &word_ptr(b, s)->x;
+    const float* q1 = &word_ptr(b + V, s)->x;
     return alg8<float>{{__builtin_nontemporal_load(q0), __builtin_nontemporal_load(q0 + 1),
                         __builtin_nontemporal_load(q0 + 2), __builtin_nontemporal_load(q0 + 3),
                         __builtin_nontemporal_load(q1), __builtin_nontemporal_load(q1 + 1),
                         __builtin_nontemporal_load(q1 + 2), __builtin_nontemporal_load(q1 + 3)}};
 #else
-    const float4 a = b[0], c = b[V];
+    const float4 a = word_at(b, s), c = word_at(b + V, s);
     return alg8<float>{{a.x, a.y, a.z, a.w, c.x, c.y, c.z, c.w}};
 #endif
 }
 __device__ inline void store_alg32(float4* __restrict__ p, unsigned V, int mu, unsigned s,
                                    const alg8<float>& a) {
-    float4* b = p + (size_t)mu * 2 * V + s;
-    put4(b, make_float4(a.a[0], a.a[1], a.a[2], a.a[3]));
-    put4(b + V, make_float4(a.a[4], a.a[5], a.a[6], a.a[7]));
+    float4* b = p + (size_t)mu * 2 * V;
+    put4(word_ptr(b, s), make_float4(a.a[0], a.a[1], a.a[2], a.a[3]));
+    put4(word_ptr(b + V, s), make_float4(a.a[4], a.a[5], a.a[6], a.a[7]));
 }
 
 // ---- storage traits used by the generic kernels ------------------------------

@@ -16,17 +16,33 @@ namespace qcb {
 //   upper = U_nu(x+mu) (U_nu(x) U_mu(x+nu))^dag
 //   lower = (U_mu(x-nu) U_nu(x+mu-nu))^dag U_nu(x-nu)
 // (the two terms of GaugeField::compute_staple, src/gauge/GaugeField.cpp:74-97)
+// *_rows: rows 0-1 only (the staple is in SU(3); its third row is rebuilt
+// where it is needed, su3.hpp).
+template <class F, int MU>
+__device__ inline mat3<typename F::real> upper_rows(const F& f, unsigned s, const nbr& n,
+                                                    int nu) {
+    const unsigned xmu = s + n.up[MU], xnu = s + n.up[nu];
+    return mul_nd_rows(f.link(nu, xmu), mul_su3(f.link(nu, s), f.link(MU, xnu)));
+}
+template <class F, int MU>
+__device__ inline mat3<typename F::real> lower_rows(const F& f, unsigned s, const nbr& n,
+                                                    int nu) {
+    const unsigned xmnu = s + n.dn[nu], xmunu = s + n.up[MU] + n.dn[nu];
+    return mul_dn_rows(mul_su3(f.link(MU, xmnu), f.link(nu, xmunu)), f.link(nu, xmnu));
+}
 template <class F, int MU>
 __device__ inline mat3<typename F::real> upper_staple(const F& f, unsigned s, const nbr& n,
                                                       int nu) {
-    const unsigned xmu = s + n.up[MU], xnu = s + n.up[nu];
-    return mul_nd_su3(f.link(nu, xmu), mul_su3(f.link(nu, s), f.link(MU, xnu)));
+    mat3<typename F::real> w = upper_rows<F, MU>(f, s, n, nu);
+    complete_su3(w);
+    return w;
 }
 template <class F, int MU>
 __device__ inline mat3<typename F::real> lower_staple(const F& f, unsigned s, const nbr& n,
                                                       int nu) {
-    const unsigned xmnu = s + n.dn[nu], xmunu = s + n.up[MU] + n.dn[nu];
-    return mul_dn_su3(mul_su3(f.link(MU, xmnu), f.link(nu, xmunu)), f.link(nu, xmnu));
+    mat3<typename F::real> w = lower_rows<F, MU>(f, s, n, nu);
+    complete_su3(w);
+    return w;
 }
 
 // The link words of lane src (the words a load at that lane's site returns).
@@ -56,6 +72,25 @@ struct no_staple_hook {
     __device__ void lower(int, const M&) {}
 };
 
+// Adds the staple w (rows 0-1 given) to the staple sum st (st = w for the
+// first), and hands the whole staple to the hook.  The sum forms the third
+// row of w only inside its own multiply-add chain (add_su3_rows); a hook that
+// needs it gets it rebuilt on its own, so every kernel sums the staples with
+// the same roundings, hook or not.
+template <bool UPPER, class Hook, typename T>
+__device__ inline void take_staple(mat3<T>& st, const mat3<T>& w, bool first, int nu,
+                                   Hook&& hook) {
+    sum_su3_rows(st, w, first);
+    if constexpr (!std::is_same<typename std::decay<Hook>::type, no_staple_hook>::value) {
+        mat3<T> full = w;
+        complete_su3(full);
+        if constexpr (UPPER)
+            hook.upper(nu, full);
+        else
+            hook.lower(nu, full);
+    }
+}
+
 // Staple sum of U_mu(x):
 //   sum_{nu != mu} U_nu(x+mu) (U_nu(x) U_mu(x+nu))^dag
 //                + (U_mu(x-nu) U_nu(x+mu-nu))^dag U_nu(x-nu)
@@ -70,6 +105,7 @@ template <class F, int MU, bool XROWS = false, class Hook = no_staple_hook>
 __device__ inline mat3<typename F::real> staple_sum(const F& f, unsigned s, const nbr& n,
                                                     Hook&& hook = Hook{}) {
     using T = typename F::real;
+    // (set by the first staple on the SU(3)-product path)
     mat3<T> st = zero3<T>();
     const unsigned xmu = s + n.up[MU];
 #pragma unroll
@@ -124,52 +160,43 @@ __device__ inline mat3<typename F::real> staple_sum(const F& f, unsigned s, cons
             }
         } else if constexpr (QCB_SU3_PRODUCTS) {
             // every product of links is in SU(3): form two rows, rebuild the third
+            // the first staple of the sum (the upper one of the first plane)
+            // sets it
+            const bool first = nu == (MU == 0 ? 1 : 0);
             if (XROWS && (MU == 0 || nu == 0)) {
                 // The x-neighbours of this plane's links are in this wavefront:
                 // shuffle them instead of loading them (same words, same
-                // products as upper_staple / lower_staple).
+                // products as upper_rows / lower_rows).
                 using W = typename F::word;
                 if (MU == 0) {  // U_nu(x+mu) is lane xp's U_nu(x)
                     W a[F::kWords], ap[F::kWords];
                     f.load_words(nu, s, a);
                     shfl_words<F>(a, n.xp, ap);
-                    const mat3<T> wu = mul_nd_su3(F::from_words(ap),
-                                                  mul_su3(F::from_words(a), f.link(MU, xnu)));
-                    add_to(st, wu);
-                    hook.upper(nu, wu);
-                    const mat3<T> wd = lower_staple<F, MU>(f, s, n, nu);
-                    add_to(st, wd);
-                    hook.lower(nu, wd);
+                    take_staple<true>(st,
+                                      mul_nd_rows(F::from_words(ap),
+                                                  mul_su3(F::from_words(a), f.link(MU, xnu))),
+                                      first, nu, hook);
+                    take_staple<false>(st, lower_rows<F, MU>(f, s, n, nu), false, nu, hook);
                 } else {  // nu == 0: U_mu(x+nu), U_mu(x-nu), U_nu(x-nu) from lanes
                     W own[F::kWords], a[F::kWords], t[F::kWords];
                     f.load_words(MU, s, own);
                     f.load_words(nu, s, a);
                     shfl_words<F>(own, n.xp, t);
-                    const mat3<T> wu = mul_nd_su3(f.link(nu, xmu),
-                                                  mul_su3(F::from_words(a), F::from_words(t)));
-                    add_to(st, wu);
-                    hook.upper(nu, wu);
+                    take_staple<true>(st,
+                                      mul_nd_rows(f.link(nu, xmu),
+                                                  mul_su3(F::from_words(a), F::from_words(t))),
+                                      first, nu, hook);
                     shfl_words<F>(own, n.xm, t);
                     const mat3<T> b = mul_su3(F::from_words(t), f.link(nu, xmunu));
                     shfl_words<F>(a, n.xm, t);
-                    const mat3<T> wd = mul_dn_su3(b, F::from_words(t));
-                    add_to(st, wd);
-                    hook.lower(nu, wd);
+                    take_staple<false>(st, mul_dn_rows(b, F::from_words(t)), false, nu, hook);
                 }
             } else {
-                {
-                    const mat3<T> wu = upper_staple<F, MU>(f, s, n, nu);
-                    add_to(st, wu);
-                    hook.upper(nu, wu);
-                }
+                take_staple<true>(st, upper_rows<F, MU>(f, s, n, nu), first, nu, hook);
 #if QCB_SCHED_SPLIT >= 2
                 if constexpr (sizeof(T) == 8) __builtin_amdgcn_sched_barrier(0);
 #endif
-                {
-                    const mat3<T> wd = lower_staple<F, MU>(f, s, n, nu);
-                    add_to(st, wd);
-                    hook.lower(nu, wd);
-                }
+                take_staple<false>(st, lower_rows<F, MU>(f, s, n, nu), false, nu, hook);
             }
         } else {
             {

@@ -74,7 +74,7 @@ __device__ inline mat3<typename storage<P>::real> staple_sum_lds(
     const typename storage<P>::field& f, unsigned s, const nbr& n, const lds_link<P>* own,
     const lds_link<P>* dn, unsigned lane) {
     using T = typename storage<P>::real;
-    mat3<T> st = zero3<T>();
+    mat3<T> st;
     const unsigned xmu = s + n.up[MU];
 #pragma unroll
     for (int nu = 0; nu < 4; nu++) {
@@ -82,11 +82,11 @@ __device__ inline mat3<typename storage<P>::real> staple_sum_lds(
         const unsigned xnu = s + n.up[nu];
         const unsigned xmnu = s + n.dn[nu];
         const unsigned xmunu = xmu + n.dn[nu];
-        {  // U_nu(x+mu) (U_nu(x) U_mu(x+nu))^dag
+        {  // U_nu(x+mu) (U_nu(x) U_mu(x+nu))^dag; the first staple sets the sum
             const mat3<T> c = (XROWS && nu == 0) ? from_lds<P>(own[MU], n.xp) : f.link(MU, xnu);
             const mat3<T> b = mul_su3(from_lds<P>(own[nu], lane), c);
             const mat3<T> a = (XROWS && MU == 0) ? from_lds<P>(own[nu], n.xp) : f.link(nu, xmu);
-            add_to(st, mul_nd_su3(a, b));
+            sum_su3_rows(st, mul_nd_rows(a, b), nu == (MU == 0 ? 1 : 0));
         }
         {  // (U_mu(x-nu) U_nu(x+mu-nu))^dag U_nu(x-nu)
             const mat3<T> d = (XROWS && nu == 0) ? from_lds<P>(own[MU], n.xm) : f.link(MU, xmnu);
@@ -94,7 +94,7 @@ __device__ inline mat3<typename storage<P>::real> staple_sum_lds(
             const mat3<T> b = mul_su3(d, e);
             const mat3<T> v =
                 (XROWS && nu == 0) ? from_lds<P>(own[0], n.xm) : from_lds<P>(dn[nu], lane);
-            add_to(st, mul_dn_su3(b, v));
+            add_su3_rows(st, mul_dn_rows(b, v));
         }
         // keep the next plane's loads below this plane's arithmetic (FP64; the
         // FP32-mixed code is scheduled freely, as in the plain kernel)

@@ -202,9 +202,9 @@ __host__ __device__ inline mat3<T> mul_su3(const mat3<T>& A, const mat3<T>& B) {
     complete_su3(R);
     return R;
 }
-// A B^dag, A and B in SU(3) (reads rows 0-1 of A)
+// Rows 0-1 of A B^dag (row 2 of the result is left unset; reads rows 0-1 of A)
 template <typename T>
-__host__ __device__ inline mat3<T> mul_nd_su3(const mat3<T>& A, const mat3<T>& B) {
+__host__ __device__ inline mat3<T> mul_nd_rows(const mat3<T>& A, const mat3<T>& B) {
     mat3<T> R;
 #pragma unroll
     for (int i = 0; i < 2; i++)
@@ -215,12 +215,11 @@ __host__ __device__ inline mat3<T> mul_nd_su3(const mat3<T>& A, const mat3<T>& B
             s = madd_c(s, A.e[i][2], B.e[j][2]);
             R.e[i][j] = s;
         }
-    complete_su3(R);
     return R;
 }
-// A^dag B, A and B in SU(3)
+// Rows 0-1 of A^dag B (row 2 of the result is left unset)
 template <typename T>
-__host__ __device__ inline mat3<T> mul_dn_su3(const mat3<T>& A, const mat3<T>& B) {
+__host__ __device__ inline mat3<T> mul_dn_rows(const mat3<T>& A, const mat3<T>& B) {
     mat3<T> R;
 #pragma unroll
     for (int i = 0; i < 2; i++)
@@ -231,8 +230,57 @@ __host__ __device__ inline mat3<T> mul_dn_su3(const mat3<T>& A, const mat3<T>& B
             s = madd_cl(s, A.e[2][i], B.e[2][j]);
             R.e[i][j] = s;
         }
+    return R;
+}
+// A B^dag, A and B in SU(3) (reads rows 0-1 of A)
+template <typename T>
+__host__ __device__ inline mat3<T> mul_nd_su3(const mat3<T>& A, const mat3<T>& B) {
+    mat3<T> R = mul_nd_rows(A, B);
     complete_su3(R);
     return R;
+}
+// A^dag B, A and B in SU(3)
+template <typename T>
+__host__ __device__ inline mat3<T> mul_dn_su3(const mat3<T>& A, const mat3<T>& B) {
+    mat3<T> R = mul_dn_rows(A, B);
+    complete_su3(R);
+    return R;
+}
+// Sums of SU(3) matrices given by their rows 0-1 (W's row 2 is conj(row0 x row1)):
+//   set_su3_rows: S = W;   add_su3_rows: S += W,
+// the cross product entering S's third row inside its multiply-add chain, so
+// the third row of W is never formed and added on its own.
+template <typename T>
+__host__ __device__ inline void set_su3_rows(mat3<T>& S, const mat3<T>& W) {
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) S.e[i][j] = W.e[i][j];
+    complete_su3(S);
+}
+// acc + conj(a b - c d), with acc first in the chain
+template <typename T>
+__host__ __device__ inline cpx<T> add_conj_cross(cpx<T> acc, cpx<T> a, cpx<T> b, cpx<T> c,
+                                                 cpx<T> d) {
+    return {acc.r + a.r * b.r - a.i * b.i - c.r * d.r + c.i * d.i,
+            acc.i - a.r * b.i - a.i * b.r + c.r * d.i + c.i * d.r};
+}
+template <typename T>
+__host__ __device__ inline void add_su3_rows(mat3<T>& S, const mat3<T>& W) {
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) S.e[i][j] = S.e[i][j] + W.e[i][j];
+    S.e[2][0] = add_conj_cross(S.e[2][0], W.e[0][1], W.e[1][2], W.e[0][2], W.e[1][1]);
+    S.e[2][1] = add_conj_cross(S.e[2][1], W.e[0][2], W.e[1][0], W.e[0][0], W.e[1][2]);
+    S.e[2][2] = add_conj_cross(S.e[2][2], W.e[0][0], W.e[1][1], W.e[0][1], W.e[1][0]);
+}
+template <typename T>
+__host__ __device__ inline void sum_su3_rows(mat3<T>& S, const mat3<T>& W, bool first) {
+    if (first)
+        set_su3_rows(S, W);
+    else
+        add_su3_rows(S, W);
 }
 template <typename T>
 __host__ __device__ inline void add_to(mat3<T>& R, const mat3<T>& A) {
@@ -331,13 +379,21 @@ __host__ __device__ inline void horner_n(H t, H d, cpx<H>& q0, cpx<H>& q1, cpx<H
 #define QCB_HORNER_SHORT 1
 #endif
 constexpr double kHornerShortT = 0.01;
+// Where exp(X) is rounded to FP32 (the FP32-mixed flow), below this t the
+// terms n = 8..18 change it by less than 2^-41: |X| < sqrt(2t) = 0.1 and
+// sum_{n>7} |X|^n/n! < |X|^8/8! e^|X| < 2.8e-13, 1/2^17 of an FP32 ulp of 1.
+// The coefficients are still evaluated in FP64, from the degree-7 polynomial.
+constexpr double kHornerF32T = 0.005;
 
 // The coefficients of the 18-term series, evaluated as the degree-10
-// polynomial where that is equal to it within 2^-56 (per lane, so the
-// choice never depends on which sites share a wavefront).
-template <typename H>
+// polynomial where that is equal to it within 2^-56, and for an FP32 result
+// (F32OUT) as the degree-7 one where that is equal to it within 2^-41 (per
+// lane, so the choice never depends on which sites share a wavefront).
+template <typename H, bool F32OUT = false>
 __host__ __device__ inline void horner_exp(H t, H d, cpx<H>& q0, cpx<H>& q1, cpx<H>& q2) {
-    if (QCB_HORNER_SHORT && t < H(kHornerShortT))
+    if (F32OUT && QCB_HORNER_SHORT && t < H(kHornerF32T))
+        horner_n<7>(t, d, q0, q1, q2);
+    else if (QCB_HORNER_SHORT && t < H(kHornerShortT))
         horner_n<10>(t, d, q0, q1, q2);
     else
         horner_n<18>(t, d, q0, q1, q2);
@@ -377,7 +433,7 @@ __host__ __device__ inline mat3<T> exp_su3(const alg8<T>& x, bool* warn) {
            H(2) * (pri * H(q.r) - prr * H(q.i));
     if (warn && tt > H(1)) *warn = true;
     cpx<H> h0, h1, h2;
-    horner_exp<H>(tt, dd, h0, h1, h2);
+    horner_exp<H, sizeof(T) == 4>(tt, dd, h0, h1, h2);
     const cpx<T> q0 = {T(h0.r), T(h0.i)}, q1 = {T(h1.r), T(h1.i)}, q2 = {T(h2.r), T(h2.i)};
 
     mat3<T> E;

@@ -50,7 +50,9 @@ def test_plaquette_at_beta_6_with_error():
     bins = ps.reshape(20, 100).mean(axis=1)
     P, err = ps.mean(), bins.std(ddof=1) / np.sqrt(len(bins))
     say(f"  beta 6.0, 16^4: <P> = {P:.6f} +- {err:.6f} (20 bins of 100 sweeps)")
-    assert err < 3e-5
+    # the error is about 3e-5; its 20-bin estimate itself scatters by ~16%
+    # (2.5e-5 to 3.8e-5 over builds whose staple sums round differently)
+    assert err < 5e-5
     assert abs(P - 0.59368) < 4 * err + 5e-5
 
 
