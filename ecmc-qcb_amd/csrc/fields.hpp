@@ -88,12 +88,13 @@ struct field32c {
     }
 };
 
+// One 16-byte store per lane: the word goes out as a vector, so the store
+// never splits into narrower ones (which write their cache lines in parts).
+typedef float float4_vec __attribute__((ext_vector_type(4)));
 __device__ inline void put4(float4* p, float4 v) {
 #if QCB_NT_STORES
-    __builtin_nontemporal_store(v.x, &p->x);
-    __builtin_nontemporal_store(v.y, &p->y);
-    __builtin_nontemporal_store(v.z, &p->z);
-    __builtin_nontemporal_store(v.w, &p->w);
+    const float4_vec w = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(w, reinterpret_cast<float4_vec*>(p));
 #else
     *p = v;
 #endif

@@ -340,10 +340,12 @@ struct field64 {
 // Stores of the flow's output fields are non-temporal: the next stage reads
 // them back from HBM anyway, and keeping them out of L2 leaves it to the
 // neighbour links the current stage re-reads.
+// One 16-byte store per lane (a vector store never splits into narrower ones).
+typedef double double2_vec __attribute__((ext_vector_type(2)));
 __device__ inline void put2(double2* p, double2 v) {
 #if QCB_NT_STORES
-    __builtin_nontemporal_store(v.x, &p->x);
-    __builtin_nontemporal_store(v.y, &p->y);
+    const double2_vec w = {v.x, v.y};
+    __builtin_nontemporal_store(w, reinterpret_cast<double2_vec*>(p));
 #else
     *p = v;
 #endif
