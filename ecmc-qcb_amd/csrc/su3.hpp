@@ -358,13 +358,27 @@ __host__ __device__ inline void horner_n(H t, H d, cpx<H>& q0, cpx<H>& q1, cpx<H
     static_assert(N >= 1 && N <= 18, "degree 1..18");
     constexpr horner_coeffs hc = make_horner_coeffs();
     cpx<H> a0 = {H(hc.c[N]), H(0)}, a1 = {H(0), H(0)}, a2 = {H(0), H(0)};
+    // The first iterations multiply parts that are exactly zero (a1 and a2
+    // start at 0; their imaginary parts fill in only from the fourth and
+    // fifth iteration on).  The flags track those parts, so the products
+    // with them are left out; for finite t and d that changes no value (c +
+    // d*0 == c), and a fully unrolled loop folds the flags away.
+    bool z0i = true, z1r = true, z1i = true, z2r = true, z2i = true;
 #pragma unroll
     for (int n = N - 1; n >= 0; n--) {
-        cpx<H> n0 = {H(hc.c[n]) + d * a2.i, -d * a2.r};  // c_n - i d q2
-        cpx<H> n1 = {a0.r - t * a2.r, a0.i - t * a2.i};   // q0 - t q2
+        // c_n - i d q2
+        const cpx<H> n0 = {z2i ? H(hc.c[n]) : H(hc.c[n]) + d * a2.i, z2r ? H(0) : -d * a2.r};
+        // q0 - t q2
+        const cpx<H> n1 = {z2r ? a0.r : a0.r - t * a2.r, z2i ? a0.i : a0.i - t * a2.i};
+        const bool zn0i = z2r, zn1i = z0i && z2i;
         a2 = a1;
+        z2r = z1r;
+        z2i = z1i;
         a1 = n1;
+        z1r = false;
+        z1i = zn1i;
         a0 = n0;
+        z0i = zn0i;
     }
     q0 = a0;
     q1 = a1;
