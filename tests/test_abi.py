@@ -36,3 +36,18 @@ def test_library_is_gfx950_code_object():
     with open(qcbflow.LIB_PATH, "rb") as f:
         blob = f.read()
     assert b"gfx950" in blob
+
+
+def test_create_refuses_volumes_of_2_28_sites():
+    """qcbf_create and qcbf_batch_create refuse 2^28 sites or more before any
+    HIP call (no GPU needed): the FP32-mixed kernels address a field's word
+    arrays by 32-bit byte offsets of 16 bytes per site."""
+    lib = qcbflow.load()
+    h = ctypes.c_void_p()
+    for prec in (qcbflow.FP64, qcbflow.FP32_MIXED):
+        dims = (ctypes.c_int * 4)(128, 128, 128, 128)
+        assert lib.qcbf_create(dims, prec, ctypes.byref(h)) == qcbflow.QCBF_ERR_ARG
+        assert not h.value
+        both = (ctypes.c_int * 8)(4, 4, 4, 4, 128, 128, 128, 128)
+        assert lib.qcbf_batch_create(2, both, prec, ctypes.byref(h)) == qcbflow.QCBF_ERR_ARG
+        assert not h.value
