@@ -1,6 +1,8 @@
 // flow_body.hpp -- per-link device code of the fused RK3 stage, shared by the
 // single-lattice kernels (flow.hip) and the ensemble kernels (batch.hip).
 #pragma once
+#include <type_traits>
+
 #include "fields.hpp"
 
 namespace qcb {
@@ -105,7 +107,8 @@ template <class F, int MU, bool XROWS = false, class Hook = no_staple_hook>
 __device__ inline mat3<typename F::real> staple_sum(const F& f, unsigned s, const nbr& n,
                                                     Hook&& hook = Hook{}) {
     using T = typename F::real;
-    // (set by the first staple on the SU(3)-product path)
+    // zero for the streamed-row and full-product paths; the SU(3)-product
+    // path sets it from its first staple
     mat3<T> st = zero3<T>();
     const unsigned xmu = s + n.up[MU];
 #pragma unroll
@@ -159,9 +162,9 @@ __device__ inline mat3<typename F::real> staple_sum(const F& f, unsigned s, cons
                 }
             }
         } else if constexpr (QCB_SU3_PRODUCTS) {
-            // every product of links is in SU(3): form two rows, rebuild the third
-            // the first staple of the sum (the upper one of the first plane)
-            // sets it
+            // Every product of links is in SU(3): form two rows, rebuild the
+            // third.  The first staple (the upper one of the first plane) sets
+            // the sum.
             const bool first = nu == (MU == 0 ? 1 : 0);
             if (XROWS && (MU == 0 || nu == 0)) {
                 // The x-neighbours of this plane's links are in this wavefront:

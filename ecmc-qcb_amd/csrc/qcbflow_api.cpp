@@ -172,6 +172,8 @@ extern "C" int qcbf_create(const int dims[4], int precision, qcbf_flow** out) {
             return fail(nullptr, QCBF_ERR_ARG, "extent L[%d] = %d out of range [2, 4096]", d, dims[d]);
         V *= (unsigned long long)dims[d];
     }
+    // below 2^28 sites: the FP32-mixed kernels address by 32-bit byte offsets
+    static_assert((1ull << 28) <= qcb::kMaxSites32c, "volume limit beyond 32-bit site offsets");
     if (V >= (1ull << 28)) return fail(nullptr, QCBF_ERR_ARG, "volume %llu too large", V);
     qcbf_flow* h = new qcbf_flow();
     h->precision = precision;

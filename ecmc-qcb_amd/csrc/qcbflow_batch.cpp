@@ -159,6 +159,7 @@ extern "C" int qcbf_batch_create(int n, const int* dims, int precision, qcbf_bat
             }
             V *= (unsigned long long)L;
         }
+        static_assert((1ull << 28) <= qcb::kMaxSites32c, "volume limit beyond 32-bit site offsets");
         if (V >= (1ull << 28)) {
             delete b;
             g_batch_err = "configuration volume too large";
