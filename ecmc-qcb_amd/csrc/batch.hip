@@ -34,6 +34,20 @@ namespace qcb {
         }                                                                    \
     } while (0)
 
+// A wavefront's 64 sites are one tile of one configuration, so that
+// configuration's field offsets and volume are the same for all its lanes:
+// taken into scalar registers, the field's word bases stay wave-uniform and
+// the loads take the site as a 32-bit offset (fields.hpp: word_at) instead
+// of a 64-bit address per lane and word.
+__device__ inline size_t wave_uniform(size_t v) {
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return ((size_t)hi << 32) | lo;
+}
+__device__ inline unsigned wave_uniform(unsigned v) {
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
 template <prec P, int STAGE, int DPW>
 __device__ __forceinline__ void batch_stage_slot(unsigned v,
                                                  const typename storage<P>::vec* __restrict__ Uin,
@@ -52,8 +66,9 @@ __device__ __forceinline__ void batch_stage_slot(unsigned v,
     const unsigned s = tile_site(c.g, tile - c.tile0, threadIdx.x, co);
     if (s >= c.g.V) return;
     const nbr n = neighbours_at(c.g, co);
-    stage_directions<P, STAGE, false, DPW>(threadIdx.y, Uin + c.u_off, Uout + c.u_off,
-                                           acc + c.a_off, c.g.V, s, n, cz, ca, w);
+    const size_t uo = wave_uniform(c.u_off), ao = wave_uniform(c.a_off);
+    stage_directions<P, STAGE, false, DPW>(threadIdx.y, Uin + uo, Uout + uo, acc + ao,
+                                           wave_uniform(c.g.V), s, n, cz, ca, w);
 }
 
 // Wavefronts per SIMD of the one-workgroup-per-tile stage kernel: the
@@ -161,7 +176,7 @@ __device__ __forceinline__ void batch_measure_slot(unsigned v,
     const cfg_desc& c = cfgs[tile_cfg[tile]];
     const unsigned s = (tile - c.tile0) * kSitesPerBlock + lane;
     const bool valid = s < c.g.V;
-    const FT f = storage<P>::view(U + c.u_off, c.g.V);
+    const FT f = storage<P>::view(U + wave_uniform(c.u_off), wave_uniform(c.g.V));
     const nbr n = neighbours(c.g, valid ? s : 0u);
     double r[3] = {0.0, 0.0, 0.0};
     if (valid) {
@@ -226,10 +241,11 @@ __device__ __forceinline__ void batch_stage1_measure_slot(
     const cfg_desc& c = cfgs[tile_cfg[tile]];
     const unsigned s = (tile - c.tile0) * kSitesPerBlock + lane;
     const bool valid = s < c.g.V;
-    const FT f = storage<P>::view(Uin + c.u_off, c.g.V);
+    const size_t uo = wave_uniform(c.u_off);
+    const FT f = storage<P>::view(Uin + uo, wave_uniform(c.g.V));
     const nbr n = neighbours(c.g, valid ? s : 0u);
-    auto* out = Uout + c.u_off;
-    auto* a = acc + c.a_off;
+    auto* out = Uout + uo;
+    auto* a = acc + wave_uniform(c.a_off);
     double r[3] = {0.0, 0.0, 0.0};
     if (valid) {
         switch (threadIdx.y) {

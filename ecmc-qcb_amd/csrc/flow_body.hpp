@@ -196,9 +196,12 @@ __device__ inline mat3<typename F::real> staple_sum(const F& f, unsigned s, cons
                 }
             } else {
                 take_staple<true>(st, upper_rows<F, MU>(f, s, n, nu), first, nu, hook);
-#if QCB_SCHED_SPLIT >= 2
-                if constexpr (sizeof(T) == 8) __builtin_amdgcn_sched_barrier(0);
-#endif
+                // FP64 without x-row shuffles: the lower staple's loads stay
+                // below the upper one's arithmetic too, which keeps the plain
+                // and ensemble kernels within three wavefronts' registers
+                // (150-158 VGPRs, against 168 and 12-23 spilled without)
+                if constexpr (sizeof(T) == 8 && (QCB_SCHED_SPLIT >= 2 || !XROWS))
+                    __builtin_amdgcn_sched_barrier(0);
                 take_staple<false>(st, lower_rows<F, MU>(f, s, n, nu), false, nu, hook);
             }
         } else {

@@ -147,7 +147,10 @@ k_rk3_stage_lds(const typename storage<P>::vec* __restrict__ Uin,
 #pragma unroll
             for (int nu = 1; nu < 4; nu++) stage_to_lds<P>(Uin, g.V, nu, s + n.dn[nu], dn[nu]);
     } else {
-        stage_to_lds<P>(Uin, g.V, mu, s + n.dn[mu], dn[mu]);
+        // n.dn[mu] by selects: indexed by the run-time mu the array would
+        // go to scratch memory (44 bytes per lane written and read back)
+        const int dmu = mu == 0 ? n.dn[0] : mu == 1 ? n.dn[1] : mu == 2 ? n.dn[2] : n.dn[3];
+        stage_to_lds<P>(Uin, g.V, mu, s + dmu, dn[mu]);
     }
     __builtin_amdgcn_s_waitcnt(0x3f70);  // vmcnt(0): the LDS copies have landed
     __syncthreads();

@@ -267,6 +267,12 @@ __host__ __device__ inline cpx<T> add_conj_cross(cpx<T> acc, cpx<T> a, cpx<T> b,
 }
 template <typename T>
 __host__ __device__ inline void add_su3_rows(mat3<T>& S, const mat3<T>& W) {
+#if defined(QCB_TRY_CLASSIC)
+    mat3<T> F = W;
+    complete_su3(F);
+    add_to(S, F);
+    return;
+#endif
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
