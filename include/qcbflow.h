@@ -248,9 +248,8 @@ enum qcbf_tune_key {
                                     the links the four direction wavefronts of a tile share
                                     staged in LDS, 0 the plain stage kernel; same results
                                     bit for bit.  Applies with 64-consecutive-site tiles,
-                                    4 threads per site and a volume divisible by 64, and
-                                    for FP32-mixed when Lx divides 64 (the plain kernel
-                                    runs otherwise). */
+                                    4 threads per site and a volume divisible by 64 (the
+                                    plain kernel runs otherwise). */
 };
 /* Set one launch parameter of h; results do not depend on it (only the order in
  * which workgroups visit sites changes).  QCBF_ERR_ARG for an unknown key or

@@ -3,9 +3,8 @@
 forms: the same products on the same words in the same order, so fields and
 measured series must be bit-identical.  Lattices with whole x-rows per
 wavefront (x-neighbours read from LDS) and without (Lx not dividing 64: the
-FP64 form still stages the same-site links, the FP32-mixed form runs the plain
-kernel), and one whose volume is not a multiple of 64 (the plain kernel
-runs)."""
+same-site links are still staged), and one whose volume is not a multiple of
+64 (the plain kernel runs)."""
 import numpy as np
 import pytest
 
