@@ -13,6 +13,7 @@
 #include "kernels.hpp"
 #include "measure_body.hpp"
 #include "reduce.hpp"
+#include "stage_lds.hpp"
 
 namespace qcb {
 
@@ -95,6 +96,46 @@ __attribute__((amdgpu_waves_per_eu(PERSIST ? 1 : kFlatStageWaves<P>))) void k_ba
     if (w) atomicOr(warn, 1);
 }
 
+// One workgroup per 64-site tile with the links its four direction
+// wavefronts share staged in LDS (k_rk3_stage_lds without whole x-rows, so
+// the same body for every member): needs every member's volume a multiple of
+// 64 with 64-consecutive-site tiles, one thread per link and the flat grid.
+// The products are those of the plain body in the same order, so the flow
+// stays bit-identical to it and to the single-lattice path.
+template <prec P, int STAGE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kLdsStageWaves<P>))) void
+k_batch_stage_lds(const typename storage<P>::vec* __restrict__ Uin,
+                  typename storage<P>::vec* __restrict__ Uout,
+                  typename storage<P>::avec* __restrict__ acc, const cfg_desc* __restrict__ cfgs,
+                  const unsigned* __restrict__ tile_cfg, unsigned ntiles, int map_mode,
+                  unsigned map_win, double cz, double ca, int* __restrict__ warn) {
+    __shared__ lds_link<P> own[4], dn[4];
+    const unsigned tile = tile_of_block(blockIdx.x, ntiles, map_mode, map_win);
+    if (tile >= ntiles) return;  // the whole workgroup: no barrier is skipped
+    const cfg_desc& c = cfgs[tile_cfg[tile]];
+    const unsigned lane = threadIdx.x;
+    const int mu = (int)threadIdx.y;  // wavefront-uniform
+    unsigned co[4];
+    const unsigned s = tile_site(c.g, tile - c.tile0, lane, co);  // full tiles: s < V
+    const nbr n = neighbours_at(c.g, co);
+    const size_t uo = wave_uniform(c.u_off), ao = wave_uniform(c.a_off);
+    const unsigned V = wave_uniform(c.g.V);
+    stage_tile_to_lds<P, false>(Uin + uo, V, mu, s, n, own, dn);
+    __builtin_amdgcn_s_waitcnt(0x3f70);  // vmcnt(0): the LDS copies have landed
+    __syncthreads();
+    const auto f = storage<P>::view(Uin + uo, V);
+    auto* out = Uout + uo;
+    auto* a = acc + ao;
+    bool w = false;
+    switch (mu) {
+        case 0: stage_lds_body<P, STAGE, 0, false>(f, out, a, s, n, own, dn, lane, cz, ca, w); break;
+        case 1: stage_lds_body<P, STAGE, 1, false>(f, out, a, s, n, own, dn, lane, cz, ca, w); break;
+        case 2: stage_lds_body<P, STAGE, 2, false>(f, out, a, s, n, own, dn, lane, cz, ca, w); break;
+        default: stage_lds_body<P, STAGE, 3, false>(f, out, a, s, n, own, dn, lane, cz, ca, w); break;
+    }
+    if (w) atomicOr(warn, 1);
+}
+
 template <prec P, int STAGE, int DPW>
 static void batch_stage_kernel(const batch_launch& bl, hipStream_t st,
                                const typename storage<P>::vec* in, typename storage<P>::vec* out,
@@ -118,6 +159,12 @@ static hipError_t batch_stage_kernel(const batch_launch& bl, hipStream_t st,
                                      typename storage<P>::vec* out, typename storage<P>::avec* a,
                                      const cfg_desc* cfgs, const unsigned* tile_cfg,
                                      unsigned ntiles, double cz, double ca, int* warn) {
+    if (bl.lds) {  // flat grid, one thread per link (the host checks)
+        k_batch_stage_lds<P, STAGE>
+            <<<grid_for_tiles(ntiles, bl.map_mode, bl.map_win), dim3(kSitesPerBlock, 4), 0, st>>>(
+                in, out, a, cfgs, tile_cfg, ntiles, bl.map_mode, bl.map_win, cz, ca, warn);
+        return hipSuccess;
+    }
     switch (bl.dpw) {
         case 1: batch_stage_kernel<P, STAGE, 1>(bl, st, in, out, a, cfgs, tile_cfg, ntiles, cz, ca, warn); break;
         case 2: batch_stage_kernel<P, STAGE, 2>(bl, st, in, out, a, cfgs, tile_cfg, ntiles, cz, ca, warn); break;

@@ -27,6 +27,9 @@ struct batch_launch {
     int map_mode;       // tile order (tile_of_block)
     unsigned map_win;
     int dpw;            // directions per wavefront of the stage kernels (1, 2, 4)
+    // stage kernels with each tile's shared links staged in LDS (flat grid,
+    // dpw 1, every member a whole number of 64-consecutive-site tiles)
+    int lds;
 };
 
 hipError_t launch_batch_stage(prec p, int stage, const void* Uin, void* Uout, void* acc,

@@ -44,7 +44,9 @@ struct qcbf_batch {
     unsigned* d_slot = nullptr;
     int* dwarn = nullptr;
     void* stage = nullptr;
-    qcb::batch_launch bl{0, 2, 1024, 1};
+    qcb::batch_launch bl{0, 2, 1024, 1, 0};
+    int lds_stage = 1;       // QCBF_TUNE_LDS_STAGE
+    bool full_tiles = true;  // every member's volume a multiple of 64
     int blocks_per_cu = 0;  // one workgroup per tile: the autotuner's pick on MI355X
     long tile_shape = -1;
     int cus = 256;
@@ -125,6 +127,8 @@ extern "C" const char* qcbf_batch_last_error(const qcbf_batch* b) {
 
 static void set_grid(qcbf_batch* b) {
     b->bl.grid = (unsigned)(b->cus * b->blocks_per_cu);
+    b->bl.lds = b->lds_stage && b->full_tiles && b->tile_shape == -1 && b->bl.grid == 0 &&
+                b->bl.dpw == 1;
     drop_graphs(b);
 }
 
@@ -169,6 +173,7 @@ extern "C" int qcbf_batch_create(int n, const int* dims, int precision, qcbf_bat
         cd.g = qcb::make_lattice(&dims[4 * c]);
         cd.tile0 = t0;
         cd.ntiles = (cd.g.V + qcb::kSitesPerBlock - 1) / qcb::kSitesPerBlock;
+        if (cd.g.V % qcb::kSitesPerBlock != 0) b->full_tiles = false;
         cd.u_off = uo;
         cd.a_off = ao;
         for (unsigned k = 0; k < cd.ntiles; k++) tile_cfg.push_back((unsigned)c);
@@ -549,6 +554,10 @@ extern "C" int qcbf_batch_tune(qcbf_batch* b, int key, long value) {
                 return bfail(b, QCBF_ERR_ARG, "threads per site %ld not 1, 2 or 4", value);
             b->bl.dpw = 4 / (int)value;
             break;
+        case QCBF_TUNE_LDS_STAGE:
+            if (value != 0 && value != 1) return bfail(b, QCBF_ERR_ARG, "LDS stage %ld not 0 or 1", value);
+            b->lds_stage = (int)value;
+            break;
         case QCBF_TUNE_TILE_SHAPE: {
             if (value != -1 && (value < 0 || value >= 4096))
                 return bfail(b, QCBF_ERR_ARG, "tile shape %ld out of range", value);
@@ -577,6 +586,7 @@ extern "C" int qcbf_batch_get_tune(const qcbf_batch* b, int key, long* value) {
         case QCBF_TUNE_TILE_SHAPE: *value = b->tile_shape; return QCBF_OK;
         case QCBF_TUNE_FUSED_MEASURE: *value = b->fuse_measure ? 1 : 0; return QCBF_OK;
         case QCBF_TUNE_THREADS_PER_SITE: *value = 4 / b->bl.dpw; return QCBF_OK;
+        case QCBF_TUNE_LDS_STAGE: *value = b->lds_stage; return QCBF_OK;
         default: return QCBF_ERR_ARG;
     }
 }

@@ -244,12 +244,13 @@ enum qcbf_tune_key {
                                       updates 2 or 4 directions in turn, and a workgroup
                                       takes 2 or 4 tiles).  The fused measurement stage
                                       keeps 4. */
-    QCBF_TUNE_LDS_STAGE = 6      /* single lattices: 1 (default) runs the RK3 stages with
-                                    the links the four direction wavefronts of a tile share
-                                    staged in LDS, 0 the plain stage kernel; same results
-                                    bit for bit.  Applies with 64-consecutive-site tiles,
-                                    4 threads per site and a volume divisible by 64 (the
-                                    plain kernel runs otherwise). */
+    QCBF_TUNE_LDS_STAGE = 6      /* 1 (default) runs the RK3 stages with the links the
+                                    four direction wavefronts of a tile share staged in
+                                    LDS, 0 the plain stage kernel; same results bit for
+                                    bit.  Applies with 64-consecutive-site tiles, 4
+                                    threads per site and a volume divisible by 64, for
+                                    ensembles every member's, on the one-workgroup-per-
+                                    tile grid (the plain kernel runs otherwise). */
 };
 /* Set one launch parameter of h; results do not depend on it (only the order in
  * which workgroups visit sites changes).  QCBF_ERR_ARG for an unknown key or

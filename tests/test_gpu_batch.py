@@ -181,6 +181,41 @@ def test_batch_fused_measurement_is_bit_identical(prec):
             assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("prec", [qcbflow.FP64, qcbflow.FP32_MIXED])
+def test_batch_lds_stage_is_bit_identical(prec):
+    """The ensemble stage kernel with each tile's shared links staged in LDS
+    (QCBF_TUNE_LDS_STAGE = 1, the default where every member is whole 64-site
+    tiles on the one-workgroup-per-tile grid) against the plain one, graphs on
+    and off, and against the single-lattice path; an ensemble with a member
+    that is not whole tiles runs the plain kernel."""
+    for dims in ([(4, 4, 4, 4), (8, 8, 8, 8), (12, 4, 4, 4), (6, 4, 8, 4)],
+                 [(4, 4, 4, 4), (3, 5, 4, 7)]):
+        fs = fields_for(dims, sweeps=1)
+        out = []
+        for lds, graph in [(1, True), (0, True), (1, False), (0, False)]:
+            with qcbflow.Ensemble(dims, 0.02, prec) as ens:
+                ens.tune(qcbflow.TUNE_BLOCKS_PER_CU, 0)
+                ens.tune(qcbflow.TUNE_LDS_STAGE, lds)
+                assert ens.get_tune(qcbflow.TUNE_LDS_STAGE) == lds
+                ens.use_graph(graph)
+                for c, f in enumerate(fs):
+                    ens.copy(c, f)
+                ens.rk3_step(2)
+                out.append(ens.flow_measure(5, 1) + tuple(ens.field(c) for c in range(len(dims))))
+        for r in out[1:]:
+            for x, y in zip(out[0], r):
+                assert np.array_equal(x, y)
+        for c, (L, f) in enumerate(zip(dims, fs)):
+            with qcbflow.GradientFlow(L, 0.02, prec) as gf:
+                gf.copy(f)
+                gf.rk3_step(2)
+                tqe, plaq = gf.flow_measure(5, 1)
+            assert np.array_equal(tqe, out[0][0][c]) and np.array_equal(plaq, out[0][1][c])
+    with qcbflow.Ensemble([(4, 4, 4, 4)], 0.02, prec) as ens:
+        with pytest.raises(qcbflow.QcbfError):
+            ens.tune(qcbflow.TUNE_LDS_STAGE, 2)
+
+
 def test_batch_errors():
     with pytest.raises(qcbflow.QcbfError) as ei:
         qcbflow.Ensemble([(4, 4, 4, 4), (4, 1, 4, 4)])
