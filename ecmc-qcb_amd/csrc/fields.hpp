@@ -40,18 +40,6 @@ namespace qcb {
 enum class prec { f64 = 0, f32m = 1 };
 
 // ---- FP32 two-row compressed field -------------------------------------------
-// Word s of a word array of the field (16-byte words).  The array's base is
-// the same for the whole wavefront (scalar registers) and the site enters as a
-// 32-bit byte offset, so each load takes both as they are instead of a 64-bit
-// address formed per load.  Needs 16 s < 2^32: fields of at most 2^28 sites
-// (kMaxSites32c; qcbf_create and qcbf_batch_create take volumes below it).
-constexpr unsigned kMaxSites32c = 1u << 28;
-template <class W>
-__device__ inline W* word_ptr(W* base, unsigned s) {
-    static_assert(sizeof(W) == 16, "16-byte words");
-    using B = typename std::conditional<std::is_const<W>::value, const char, char>::type;
-    return reinterpret_cast<W*>(reinterpret_cast<B*>(base) + (s << 4));
-}
 __device__ inline float4 word_at(const float4* __restrict__ base, unsigned s) {
     return *word_ptr(base, s);
 }

@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "su3.hpp"
 
 namespace qcb {
@@ -280,6 +282,20 @@ __host__ __device__ inline tile_groups stage_groups(const lattice& g) {
                        (unsigned)g.dpw);
 }
 
+// Word s of a word array of a field (16-byte words).  The array's base is the
+// same for the whole wavefront (scalar registers) and the site enters as a
+// 32-bit byte offset, so each load or store takes both as they are instead of
+// a 64-bit address formed per access.  Needs 16 s < 2^32: fields of at most
+// 2^28 sites (kMaxSites32c; qcbf_create and qcbf_batch_create take volumes
+// below it).
+constexpr unsigned kMaxSites32c = 1u << 28;
+template <class W>
+__device__ inline W* word_ptr(W* base, unsigned s) {
+    static_assert(sizeof(W) == 16, "16-byte words");
+    using B = typename std::conditional<std::is_const<W>::value, const char, char>::type;
+    return reinterpret_cast<W*>(reinterpret_cast<B*>(base) + (s << 4));
+}
+
 // ---- FP64 structure-of-arrays link field -----------------------------------
 // The flow and measurement kernels read only the first two rows of every
 // FP64 link and rebuild the third as conj(row0 x row1), exact for SU(3) up to
@@ -302,9 +318,8 @@ struct field64 {
     const double2* __restrict__ p;
     unsigned V;
     __device__ inline void load_words(int mu, unsigned s, word (&w)[kWords]) const {
-        const double2* b = p + (size_t)mu * 9 * V + s;
 #pragma unroll
-        for (int k = 0; k < kWords; k++) w[k] = b[(size_t)k * V];
+        for (int k = 0; k < kWords; k++) w[k] = *word_ptr(p + ((size_t)mu * 9 + k) * V, s);
     }
     __device__ static inline mat3<double> from_words(const word (&w)[kWords]) {
         mat3<double> m;
@@ -325,10 +340,9 @@ struct field64 {
     }
     // row i of U_mu(s)
     __device__ inline void row(int mu, unsigned s, int i, cpx<double> (&r)[3]) const {
-        const double2* b = p + ((size_t)mu * 9 + 3 * i) * V + s;
 #pragma unroll
         for (int j = 0; j < 3; j++) {
-            double2 v = b[(size_t)j * V];
+            const double2 v = *word_ptr(p + ((size_t)mu * 9 + 3 * i + j) * V, s);
             r[j] = {v.x, v.y};
         }
     }
@@ -353,10 +367,10 @@ __device__ inline void put2(double2* p, double2 v) {
 
 __device__ inline void store_link64(double2* __restrict__ p, unsigned V, int mu, unsigned s,
                                     const mat3<double>& m) {
-    double2* b = p + (size_t)mu * 9 * V + s;
 #pragma unroll
     for (int k = 0; k < 9; k++)
-        put2(b + (size_t)k * V, make_double2(m.e[k / 3][k % 3].r, m.e[k / 3][k % 3].i));
+        put2(word_ptr(p + ((size_t)mu * 9 + k) * V, s),
+             make_double2(m.e[k / 3][k % 3].r, m.e[k / 3][k % 3].i));
 }
 
 // The flow kernels store rows 0 and 1 only when every reader rebuilds the
@@ -366,10 +380,10 @@ __device__ inline void store_link64(double2* __restrict__ p, unsigned V, int mu,
 constexpr bool kF64FlowStoresTwoRows = QCB_F64_RECONSTRUCT != 0;
 __device__ inline void store_flow_link64(double2* __restrict__ p, unsigned V, int mu, unsigned s,
                                          const mat3<double>& m) {
-    double2* b = p + (size_t)mu * 9 * V + s;
 #pragma unroll
     for (int k = 0; k < (kF64FlowStoresTwoRows ? 6 : 9); k++)
-        put2(b + (size_t)k * V, make_double2(m.e[k / 3][k % 3].r, m.e[k / 3][k % 3].i));
+        put2(word_ptr(p + ((size_t)mu * 9 + k) * V, s),
+             make_double2(m.e[k / 3][k % 3].r, m.e[k / 3][k % 3].i));
 }
 
 // The RK3 accumulator is read once per stage and never by a neighbour:
@@ -381,14 +395,14 @@ __device__ inline void store_flow_link64(double2* __restrict__ p, unsigned V, in
 __device__ inline alg8<double> load_alg64(const double2* __restrict__ p, unsigned V, int mu,
                                           unsigned s) {
     alg8<double> a;
-    const double2* b = p + (size_t)mu * 4 * V + s;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
+        const double2* w = word_ptr(p + ((size_t)mu * 4 + k) * V, s);
 #if QCB_NT_ACC_LOADS
-        a.a[2 * k] = __builtin_nontemporal_load(&b[(size_t)k * V].x);
-        a.a[2 * k + 1] = __builtin_nontemporal_load(&b[(size_t)k * V].y);
+        a.a[2 * k] = __builtin_nontemporal_load(&w->x);
+        a.a[2 * k + 1] = __builtin_nontemporal_load(&w->y);
 #else
-        double2 v = b[(size_t)k * V];
+        const double2 v = *w;
         a.a[2 * k] = v.x;
         a.a[2 * k + 1] = v.y;
 #endif
@@ -398,9 +412,9 @@ __device__ inline alg8<double> load_alg64(const double2* __restrict__ p, unsigne
 
 __device__ inline void store_alg64(double2* __restrict__ p, unsigned V, int mu, unsigned s,
                                    const alg8<double>& a) {
-    double2* b = p + (size_t)mu * 4 * V + s;
 #pragma unroll
-    for (int k = 0; k < 4; k++) put2(b + (size_t)k * V, make_double2(a.a[2 * k], a.a[2 * k + 1]));
+    for (int k = 0; k < 4; k++)
+        put2(word_ptr(p + ((size_t)mu * 4 + k) * V, s), make_double2(a.a[2 * k], a.a[2 * k + 1]));
 }
 
 }  // namespace qcb
