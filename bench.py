@@ -482,8 +482,9 @@ def report_ensemble(qcbflow, args):
 def ensemble_leg(qcbflow, args):
     """The 64-lattice ensemble workload (report_ensemble with its own defaults:
     150 steps, 30 sweeps, autotuned) condensed for the bench line: the
-    north_star's comparison of the FP32-mixed batched flow with the plain FP64
-    path, and the precision gates over all 64 configurations."""
+    FP32-mixed batched flow against the plain FP64 path flowing the same
+    configurations one at a time, and the precision gates over all 64
+    configurations."""
     ea = argparse.Namespace(**vars(args))
     ea.steps_given, ea.sweeps, ea.threads_per_site, ea.no_autotune = False, None, None, False
     r = report_ensemble(qcbflow, ea)
@@ -735,7 +736,7 @@ def main():
     args.steps_given = args.steps is not None
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
+        print(f"bench.py: runs on one GPU only (--gpus {args.gpus} requested); "
               "the whole lattice lives on a single MI355X.", file=sys.stderr)
         sys.exit(2)
 

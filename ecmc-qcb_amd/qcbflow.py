@@ -125,7 +125,7 @@ def load():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"{LIB_PATH} is missing: build it first (__graft_entry__.build())")
+            raise RuntimeError(f"{LIB_PATH} is missing: build it first (make -C ecmc-qcb_amd)")
         lib = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(lib, name)

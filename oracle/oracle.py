@@ -1,8 +1,8 @@
 """ctypes wrapper of the FP64 CPU oracle (qcb_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the CPU
-baseline leg of bench.py, always as the checker, never as the code measured
-or shipped.  Fields are numpy float64 arrays of shape (V, 4, 3, 3, 2) or any
+baseline leg of bench.py, always as the result the GPU path is compared
+with, never as the code measured or shipped.  Fields are numpy float64 arrays of shape (V, 4, 3, 3, 2) or any
 contiguous array with V*72 doubles, in the reference's interior order:
 site x-fastest, mu, row-major 3x3, (re, im).
 """

@@ -2,7 +2,8 @@
  * qcb_oracle.c -- FP64 CPU restatement of ECMC-QCB's Wilson-flow measurement path.
  *
  * TEST INFRASTRUCTURE ONLY.  This file is the parity oracle: the tests, the
- * smoke check and the CPU-baseline leg of the benchmark call it as the checker.
+ * smoke check and the CPU-baseline leg of the benchmark call it as the result
+ * the GPU path is compared with.
  * Nothing in the shipped library links, loads or calls it.
  *
  * Every function cites the reference file:line it restates (paths relative to
