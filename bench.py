@@ -435,6 +435,8 @@ def report_ensemble(qcbflow, args):
         m, w, b = (None, None, None)
         if args.threads_per_site is not None:
             e.tune(qcbflow.TUNE_THREADS_PER_SITE, args.threads_per_site)
+        if args.lds_stage is not None:
+            e.tune(qcbflow.TUNE_LDS_STAGE, args.lds_stage)
         if not args.no_autotune:  # the ensemble configuration is autotuned
             m, w, b = e.autotune(trial_steps=2)
         tps = e.get_tune(qcbflow.TUNE_THREADS_PER_SITE)

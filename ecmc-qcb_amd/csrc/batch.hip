@@ -72,15 +72,17 @@ __device__ __forceinline__ void batch_stage_slot(unsigned v,
                                            wave_uniform(c.g.V), s, n, cz, ca, w);
 }
 
-// Wavefronts per SIMD of the one-workgroup-per-tile stage kernel: the
-// single-lattice kernel's count, which the register allocator reaches here
-// only when asked (FP32: 76 VGPRs instead of 113).
-template <prec P>
-constexpr int kFlatStageWaves = P == prec::f64 ? 3 : 6;
+// Wavefronts per SIMD of the one-workgroup-per-tile stage kernel, which the
+// register allocator reaches here only when asked (FP32 with 2 or 4
+// directions per wavefront: 73-77 VGPRs instead of 120-129).  FP32 with one
+// direction per wavefront takes 5: at 6 (80 VGPRs) it spilled 4 dwords to
+// scratch memory and ran its ensemble 2% slower than at 5 (86 VGPRs).
+template <prec P, int DPW>
+constexpr int kFlatStageWaves = P == prec::f64 ? 3 : DPW == 1 ? 5 : 6;
 
 template <prec P, int STAGE, bool PERSIST, int DPW>
 __global__ __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(PERSIST ? 1 : kFlatStageWaves<P>))) void k_batch_stage(const typename storage<P>::vec* __restrict__ Uin,
+__attribute__((amdgpu_waves_per_eu(PERSIST ? 1 : kFlatStageWaves<P, DPW>))) void k_batch_stage(const typename storage<P>::vec* __restrict__ Uin,
                                                      typename storage<P>::vec* __restrict__ Uout,
                                                      typename storage<P>::avec* __restrict__ acc,
                                                      const cfg_desc* __restrict__ cfgs,

@@ -96,8 +96,11 @@ def _family(kernels, prefix):
     return ks
 
 
-# the kernels a default flow, ensemble, measurement or heatbath launches
+# the kernels a default flow, ensemble, measurement or heatbath launches, and
+# the ensemble stage kernels of every launch shape (the persistent fused
+# measurement, launched only by the autotuner's trials, spills 2 dwords)
 DEFAULT_PATH = ["k_rk3_stage_lds<", "k_rk3_stage<", "k_rk3_stage1_measure<", "k_batch_stage_lds<",
+                "k_batch_stage<",
                 "k_batch_stage1_measure<f64, false>", "k_batch_stage1_measure<f32m, false>",
                 "k_batch_measure<f64, false>", "k_batch_measure<f32m, false>",
                 "k_measure_clover<", "k_plaquette<", "k_reduce_measure", "k_batch_reduce",
